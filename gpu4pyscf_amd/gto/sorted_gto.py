@@ -1,0 +1,190 @@
+'''Sorted, segment-contracted Cartesian basis used by the J/K kernels.
+
+Restates gpu4pyscf/gto/mole.py:583-654 (``SortedGTO.from_mol``),
+:1157-1251 (``_optimize_contraction``), :1253-1332 (``_recontract_basis``) and
+:1120-1132 (``_scale_sp_ctr_coeff``):
+
+* every contracted shell is split into one segment of its compact primitives
+  (exponent > ``diffuse_cutoff``) plus one single-primitive shell per diffuse
+  primitive; general contractions are split as the reference does;
+* shells are grouped by (l, nprim) with l ascending and nprim descending, and
+  stably sorted into those groups;
+* all shells are Cartesian; s and p coefficients carry sqrt((2l+1)/4pi);
+* ``coeff`` maps original AOs to sorted Cartesian AOs: D_sorted = C D C^T and
+  V_orig = C^T V_sorted C.
+'''
+
+import math
+import numpy as np
+
+from gpu4pyscf_amd.gto import mole as mole_mod
+from gpu4pyscf_amd.gto.mole import (ANG_OF, NPRIM_OF, NCTR_OF, PTR_EXP, PTR_COEFF,
+                                    ATOM_OF, PTR_COORD, PTR_BAS_COORD, BAS_SLOTS,
+                                    PTR_ENV_START, PTR_RANGE_OMEGA, gto_norm)
+from gpu4pyscf_amd.gto.cart2sph import c2s
+
+
+def _split_shell(l, exps, cs, diffuse_cutoff):
+    '''Split one shell (exps[nprim], cs[nprim, nctr]) into segment shells.
+
+    Returns ``pieces`` = [(exps, coeffs)], and ``recon`` with
+    recon[ictr, ipiece] = coefficient of piece ipiece in contracted function
+    ictr (gpu4pyscf/gto/mole.py:1157-1251).'''
+    nprim, nctr = cs.shape
+    if nctr == 1:
+        c = cs[:, 0]
+        if diffuse_cutoff == 0 or np.all(exps > diffuse_cutoff):
+            return [(exps, c)], np.ones((1, 1))
+        compact = np.where(exps > diffuse_cutoff)[0]
+        diffuse = np.where(exps <= diffuse_cutoff)[0]
+        pieces = []
+        if len(compact) > 0:
+            pieces.append((exps[compact], c[compact]))
+        for p in diffuse:
+            pieces.append((exps[p:p+1], c[p:p+1]))
+        return pieces, np.ones((1, len(pieces)))
+
+    c = cs.T.copy()          # (nctr, nprim)
+    c_compact = c.copy()
+    c_compact[:, exps <= diffuse_cutoff] = 0.
+    nprim_each = np.count_nonzero(c_compact != 0, axis=1)
+    seg = np.where(nprim_each > 1)[0]
+    if len(seg) != 1:
+        # full uncontraction; every primitive is its own shell
+        norm = gto_norm(l, exps)
+        cn = c / norm
+        max_c = abs(cn).max(axis=0) + 1e-200
+        cn = cn / max_c
+        pieces = [(exps[p:p+1], np.array([norm[p] * max_c[p]])) for p in range(nprim)]
+        return pieces, cn
+    seg = seg[0]
+    rem = np.where(np.arange(nctr) != seg)[0]
+    diffuse_mask = np.any(c[rem] != 0, axis=0)
+    diffuse = np.where(diffuse_mask)[0]
+    compact = np.where(~diffuse_mask)[0]
+    norm = gto_norm(l, exps[diffuse])
+    recon = np.zeros((nctr, 1 + len(diffuse)))
+    recon[seg, 0] = 1.
+    recon[:, 1:] = c[:, diffuse] / norm
+    pieces = [(exps[compact], c[seg, compact])]
+    for k, p in enumerate(diffuse):
+        pieces.append((exps[p:p+1], np.array([norm[k]])))
+    return pieces, recon
+
+
+class SortedGTO:
+    '''Sorted Cartesian basis derived from a PySCF-layout molecule.'''
+
+    def __init__(self, mol, diffuse_cutoff=0.3):
+        self.mol = mol
+        atm = np.asarray(mol._atm, dtype=np.int32)
+        bas0 = np.asarray(mol._bas)
+        env0 = np.asarray(mol._env, dtype=np.float64)
+        cart0 = bool(getattr(mol, 'cart', False))
+
+        shells = []       # (atom, l, exps, coefs)
+        recon_entries = []  # per original shell: (shell_ids, recon matrix)
+        for ib, b in enumerate(bas0):
+            l = int(b[ANG_OF]); nprim = int(b[NPRIM_OF]); nctr = int(b[NCTR_OF])
+            exps = env0[b[PTR_EXP]:b[PTR_EXP]+nprim].copy()
+            cs = env0[b[PTR_COEFF]:b[PTR_COEFF]+nprim*nctr].reshape(nctr, nprim).T.copy()
+            pieces, recon = _split_shell(l, exps, cs, diffuse_cutoff)
+            ids = []
+            for e, c in pieces:
+                ids.append(len(shells))
+                shells.append((int(b[ATOM_OF]), l, e, c))
+            recon_entries.append((ib, l, nctr, ids, recon))
+
+        # sort by (l ascending, nprim descending), stable
+        keys = np.array([(s[1], -len(s[2])) for s in shells], dtype=np.int64).reshape(-1, 2)
+        uniq, inv, counts = np.unique(keys, axis=0, return_inverse=True, return_counts=True)
+        inv = inv.ravel()
+        order = np.argsort(inv, kind='stable')
+        uniq[:, 1] = -uniq[:, 1]
+        self.uniq_l_ctr = uniq.astype(np.int32)
+        self.l_ctr_counts = counts.astype(np.int32)
+        self.sorted_idx = order
+        new_pos = np.empty(len(shells), dtype=np.int64)
+        new_pos[order] = np.arange(len(shells))
+
+        env = env0[:PTR_ENV_START].tolist()
+        env[PTR_RANGE_OMEGA] = 0.
+        # atom coordinates
+        natm = len(atm)
+        atm = atm.copy()
+        for ia in range(natm):
+            p = mol._atm[ia, PTR_COORD]
+            atm[ia, PTR_COORD] = len(env)
+            env.extend(env0[p:p+3].tolist())
+            atm[ia, mole_mod.PTR_ZETA] = len(env)
+            env.append(env0[mol._atm[ia, mole_mod.PTR_ZETA]] if mol._atm[ia, mole_mod.PTR_ZETA] > 0 else 0.)
+        bas = np.zeros((len(shells), BAS_SLOTS), dtype=np.int32)
+        for new, old in enumerate(order):
+            ia, l, e, c = shells[old]
+            pexp = len(env); env.extend(e.tolist())
+            pcoe = len(env); env.extend(c.tolist())
+            bas[new] = [ia, l, len(e), 1, 0, pexp, pcoe, atm[ia, PTR_COORD]]
+        self._atm = atm
+        self._bas = bas
+        # _env holds the PySCF-normalised coefficients; env_kernel additionally
+        # carries the libcint s/p angular factor sqrt((2l+1)/4pi)
+        self._env = np.asarray(env, dtype=np.float64)
+        self.env_kernel = self._env.copy()
+        for b in bas:
+            if b[ANG_OF] < 2:
+                fac = math.sqrt((2*b[ANG_OF] + 1) / (4*math.pi))
+                self.env_kernel[b[PTR_COEFF]:b[PTR_COEFF]+b[NPRIM_OF]] *= fac
+        self.natm = natm
+        self.nbas = len(bas)
+        self.cart = True
+        ls = bas[:, ANG_OF]
+        self.ao_loc = np.zeros(self.nbas + 1, dtype=np.int32)
+        self.ao_loc[1:] = np.cumsum((ls + 1) * (ls + 2) // 2)
+        self.nao = int(self.ao_loc[-1])
+
+        # original AO layout
+        l0 = bas0[:, ANG_OF]
+        if cart0:
+            dims0 = (l0 + 1) * (l0 + 2) // 2 * bas0[:, NCTR_OF]
+        else:
+            dims0 = (2 * l0 + 1) * bas0[:, NCTR_OF]
+        ao_loc0 = np.zeros(len(bas0) + 1, dtype=np.int64)
+        ao_loc0[1:] = np.cumsum(dims0)
+        self.nao_orig = int(ao_loc0[-1])
+        C = np.zeros((self.nao, self.nao_orig))
+        for ib, l, nctr, ids, recon in recon_entries:
+            nfc = (l + 1) * (l + 2) // 2
+            t = np.eye(nfc) if cart0 else c2s(l)     # (nf_orig, nfc)
+            nf0 = t.shape[0]
+            for ic in range(nctr):
+                col0 = ao_loc0[ib] + ic * nf0
+                for k, sid in enumerate(ids):
+                    coef = recon[ic, k]
+                    if coef == 0:
+                        continue
+                    row0 = self.ao_loc[new_pos[sid]]
+                    C[row0:row0+nfc, col0:col0+nf0] += coef * t.T
+        self.coeff = C
+
+    @property
+    def group_offsets(self):
+        return np.append(0, np.cumsum(self.l_ctr_counts)).astype(np.int32)
+
+    def diffuse_pgto(self):
+        '''Exponent and |coefficient| of the most diffuse primitive of each
+        shell (gpu4pyscf/gto/mole.py:348-385 with op='diffuse', precision 1e-8).'''
+        exps, coefs = [], []
+        for b in self._bas:
+            l = b[ANG_OF]; n = b[NPRIM_OF]
+            e = self._env[b[PTR_EXP]:b[PTR_EXP]+n]
+            c = abs(self._env[b[PTR_COEFF]:b[PTR_COEFF]+n])
+            tmp = (2*l+1)/(4*np.pi)/1e-8**2 * c**2 * 1e2**l
+            tmp[tmp <= 0] = 1e-200
+            r2 = np.log(tmp) / (2*e)
+            k = int(np.argmax(r2))
+            exps.append(e[k]); coefs.append(c[k])
+        return np.asarray(exps), np.asarray(coefs)
+
+    def bas_coords(self):
+        p = self._bas[:, PTR_BAS_COORD]
+        return np.stack([self._env[p], self._env[p+1], self._env[p+2]], axis=1)
