@@ -1,0 +1,1094 @@
+// Direct-SCF J/K kernels for MI355X (gfx950), FP64 Rys quadrature.
+//
+// One launch evaluates one angular class (li,lj,lk,ll) of screened shell
+// quartets and contracts it with the density matrices into J and K.
+//
+// Work decomposition: a work item is (one ij shell pair, one chunk of the kl
+// pair list).  The block scans the kl candidates of its chunk, applies the
+// survival predicate of gpu4pyscf/lib/gvhf-rys/create_tasks.cu:114-180 and
+// compacts the survivors into an LDS queue with wavefront ballots; queued
+// quartets are then evaluated
+//   * by the per-lane kernel (small classes): one quartet per lane, the 2D
+//     Rys integrals and the Cartesian block all in VGPRs;
+//   * by the team kernel (large classes): a team of lanes per quartet, the
+//     2D integrals of all roots staged in LDS, each lane owning one (i,j) row
+//     of the block, the block staged through LDS for the contraction.
+// Digestion follows gpu4pyscf/lib/gvhf-rys/rys_contract_jk.cu:291-330 (four K
+// blocks and two J blocks per quartet, symmetry factors of :156-158,214-220),
+// with FP64 atomics into HBM; the J_ij block, common to a work item, is
+// reduced across the wavefront before its atomics.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include "../../include/jkhip.h"
+
+#define PI_FAC          34.98683665524972497    // 2 pi^(5/2)
+#define Q_COND_MARGIN   4.f
+#define WAVE            64
+#define RYS_DEG_MAX     15
+
+static thread_local char g_err[512];
+
+static void set_err(const char *fmt, const char *a = "", int x = 0, int y = 0)
+{
+    snprintf(g_err, sizeof(g_err), fmt, a, x, y);
+}
+
+// ---------------------------------------------------------------- helpers
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+template <int L>
+struct CartPow {
+    int x[ncart(L)], y[ncart(L)], z[ncart(L)];
+    constexpr CartPow() : x(), y(), z() {
+        int c = 0;
+        for (int lx = L; lx >= 0; --lx)
+            for (int ly = L - lx; ly >= 0; --ly) {
+                x[c] = lx; y[c] = ly; z[c] = L - lx - ly; ++c;
+            }
+    }
+};
+
+__device__ __forceinline__ void atomic_add(double *p, double v)
+{
+    unsafeAtomicAdd(p, v);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Rys nodes u_i = t_i^2 and weights on the Chebyshev tables.
+template <int N>
+__device__ __forceinline__ void rys_roots(const JKRysTable &tab, double x,
+                                          double *r, double *w)
+{
+    if (x >= tab.xmax[N-1]) {
+        const double *lr = tab.coef + tab.large_r + N * (N - 1) / 2;
+        const double *lw = tab.coef + tab.large_w + N * (N - 1) / 2;
+        double ix = 1. / x;
+        double sx = sqrt(ix);
+#pragma unroll
+        for (int i = 0; i < N; ++i) { r[i] = lr[i] * ix; w[i] = lw[i] * sx; }
+        return;
+    }
+    const int D1 = tab.degree + 1;
+    int k = (int)x;
+    double u = 2. * (x - k) - 1.;
+    double u2 = 2. * u;
+    const double *c = tab.coef + tab.off[N-1] + (int64_t)k * (2 * N) * D1;
+#pragma unroll
+    for (int f = 0; f < 2 * N; ++f) {
+        const double *cf = c + f * D1;
+        double b1 = 0., b2 = 0.;
+        for (int j = D1 - 1; j > 0; --j) {
+            double t = fma(u2, b1, cf[j] - b2);
+            b2 = b1; b1 = t;
+        }
+        double v = fma(u, b1, cf[0] - b2);
+        if (f & 1) w[f >> 1] = v; else r[f >> 1] = v;
+    }
+}
+
+// runtime-N variant (setup kernels)
+__device__ void rys_roots_rt(const JKRysTable &tab, int N, double x, double *r, double *w)
+{
+    if (x >= tab.xmax[N-1]) {
+        const double *lr = tab.coef + tab.large_r + N * (N - 1) / 2;
+        const double *lw = tab.coef + tab.large_w + N * (N - 1) / 2;
+        double ix = 1. / x, sx = sqrt(ix);
+        for (int i = 0; i < N; ++i) { r[i] = lr[i] * ix; w[i] = lw[i] * sx; }
+        return;
+    }
+    const int D1 = tab.degree + 1;
+    int k = (int)x;
+    double u = 2. * (x - k) - 1., u2 = 2. * u;
+    const double *c = tab.coef + tab.off[N-1] + (int64_t)k * (2 * N) * D1;
+    for (int f = 0; f < 2 * N; ++f) {
+        const double *cf = c + f * D1;
+        double b1 = 0., b2 = 0.;
+        for (int j = D1 - 1; j > 0; --j) {
+            double t = fma(u2, b1, cf[j] - b2);
+            b2 = b1; b1 = t;
+        }
+        double v = fma(u, b1, cf[0] - b2);
+        if (f & 1) w[f >> 1] = v; else r[f >> 1] = v;
+    }
+}
+
+// 2D Rys integrals of one Cartesian direction for one root:
+// g[((i*(LJ+1) + j)*(LK+1) + k)*(LL+1) + l], by VRR in i, TRR in k and HRR
+// for j and l (the recursions of gpu4pyscf/lib/gvhf-rys/build_rys_gxyz.cuh:
+// 18-123, evaluated in registers).
+template <int LI, int LJ, int LK, int LL>
+__device__ __forceinline__ void build_g(double *g, double g00, double c0, double cp,
+                                        double b10, double b01, double b00,
+                                        double rij, double rkl)
+{
+    constexpr int LIJ = LI + LJ, LKL = LK + LL;
+    double G[LIJ+1][LKL+1];
+    G[0][0] = g00;
+    if (LIJ > 0) G[1][0] = c0 * g00;
+#pragma unroll
+    for (int a = 1; a < LIJ; ++a)
+        G[a+1][0] = c0 * G[a][0] + (a * b10) * G[a-1][0];
+#pragma unroll
+    for (int a = 0; a <= LIJ; ++a) {
+        if (LKL > 0) {
+            double v = cp * G[a][0];
+            if (a > 0) v += (a * b00) * G[a-1][0];
+            G[a][1] = v;
+        }
+#pragma unroll
+        for (int c = 1; c < LKL; ++c) {
+            double v = cp * G[a][c] + (c * b01) * G[a][c-1];
+            if (a > 0) v += (a * b00) * G[a-1][c];
+            G[a][c+1] = v;
+        }
+    }
+    // bra HRR: H[a][b][c] = H[a+1][b-1][c] - rij H[a][b-1][c]
+    double H[LIJ+1][LJ+1][LKL+1];
+#pragma unroll
+    for (int c = 0; c <= LKL; ++c) {
+#pragma unroll
+        for (int a = 0; a <= LIJ; ++a) H[a][0][c] = G[a][c];
+#pragma unroll
+        for (int b = 0; b < LJ; ++b)
+#pragma unroll
+            for (int a = 0; a < LIJ - b; ++a)
+                H[a][b+1][c] = H[a+1][b][c] - rij * H[a][b][c];
+    }
+    // ket HRR
+#pragma unroll
+    for (int a = 0; a <= LI; ++a)
+#pragma unroll
+    for (int b = 0; b <= LJ; ++b) {
+        double T[LKL+1][LL+1];
+#pragma unroll
+        for (int c = 0; c <= LKL; ++c) T[c][0] = H[a][b][c];
+#pragma unroll
+        for (int e = 0; e < LL; ++e)
+#pragma unroll
+            for (int c = 0; c < LKL - e; ++c)
+                T[c][e+1] = T[c+1][e] - rkl * T[c][e];
+#pragma unroll
+        for (int c = 0; c <= LK; ++c)
+#pragma unroll
+            for (int e = 0; e <= LL; ++e)
+                g[((a*(LJ+1) + b)*(LK+1) + c)*(LL+1) + e] = T[c][e];
+    }
+}
+
+// ---------------------------------------------------------------- shared
+struct ShellPair {
+    int ish, jsh, i0, j0;
+    double ri[3], rij[3];      // Ri, Rj - Ri
+    int pexpi, pexpj, pci, pcj;
+};
+
+__device__ __forceinline__ void load_shell_pair(ShellPair &s, uint32_t bas_ij, int nbas,
+                                                const int32_t *bas, const double *env,
+                                                const int32_t *ao_loc)
+{
+    s.ish = bas_ij / nbas;
+    s.jsh = bas_ij - s.ish * nbas;
+    const int32_t *bi = bas + s.ish * 8, *bj = bas + s.jsh * 8;
+    const double *ri = env + bi[7], *rj = env + bj[7];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { s.ri[d] = ri[d]; s.rij[d] = rj[d] - ri[d]; }
+    s.pexpi = bi[5]; s.pexpj = bj[5]; s.pci = bi[6]; s.pcj = bj[6];
+    s.i0 = ao_loc[s.ish]; s.j0 = ao_loc[s.jsh];
+}
+
+// survival predicate, create_tasks.cu:155-170, float32 in the same order
+__device__ __forceinline__ bool survives(float q_kl, float kl_cutoff, float dm_penalty,
+                                         uint32_t bas_ij, uint32_t bas_kl, int ish, int jsh,
+                                         int nbas, const float *dm_cond)
+{
+    if (!(q_kl + dm_penalty >= kl_cutoff && bas_ij >= bas_kl)) return false;
+    int ksh = bas_kl / nbas;
+    int lsh = bas_kl - ksh * nbas;
+    float d_cutoff = kl_cutoff - q_kl;
+    return dm_cond[bas_ij] > d_cutoff ||
+           dm_cond[bas_kl] > d_cutoff ||
+           dm_cond[ish*nbas+ksh] > d_cutoff ||
+           dm_cond[jsh*nbas+ksh] > d_cutoff ||
+           dm_cond[ish*nbas+lsh] > d_cutoff ||
+           dm_cond[jsh*nbas+lsh] > d_cutoff;
+}
+
+// Work item of a launch: pair_ij index and the kl range of its chunk.
+struct WorkItem { int pij, kl0, kl1; };
+
+__device__ __forceinline__ WorkItem work_item(int npairs_kl, int kl_chunk)
+{
+    int nchunks = (npairs_kl + kl_chunk - 1) / kl_chunk;
+    WorkItem w;
+    w.pij = blockIdx.x / nchunks;
+    int ch = blockIdx.x - w.pij * nchunks;
+    w.kl0 = ch * kl_chunk;
+    w.kl1 = min(w.kl0 + kl_chunk, npairs_kl);
+    return w;
+}
+
+// ---------------------------------------------------------------- per-lane
+// One quartet per lane: returns the Cartesian block (times the symmetry and
+// normalisation prefactors) in gout[((i*nfj+j)*nfk+k)*nfl+l].
+template <int LI, int LJ, int LK, int LL>
+__device__ __forceinline__ void eri_lane(double *gout, const ShellPair &sp, int iprim, int jprim,
+                                         int kprim, int lprim, int ksh, int lsh,
+                                         const int32_t *bas, const double *env,
+                                         const JKRysTable &rys, double fac_sym)
+{
+    constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
+    constexpr int NF = NFI * NFJ * NFK * NFL;
+    constexpr int N = (LI + LJ + LK + LL) / 2 + 1;
+    constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
+    constexpr CartPow<LI> ci{};
+    constexpr CartPow<LJ> cj{};
+    constexpr CartPow<LK> ck{};
+    constexpr CartPow<LL> cl{};
+#pragma unroll
+    for (int n = 0; n < NF; ++n) gout[n] = 0.;
+    const int32_t *bk = bas + ksh * 8, *bl = bas + lsh * 8;
+    const double *rk = env + bk[7], *rl = env + bl[7];
+    double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
+    double rr_ij = sp.rij[0]*sp.rij[0] + sp.rij[1]*sp.rij[1] + sp.rij[2]*sp.rij[2];
+    double rr_kl = rkl[0]*rkl[0] + rkl[1]*rkl[1] + rkl[2]*rkl[2];
+    const double *ek = env + bk[5], *el = env + bl[5];
+    const double *cck = env + bk[6], *ccl = env + bl[6];
+    for (int kp = 0; kp < kprim; ++kp)
+    for (int lp = 0; lp < lprim; ++lp) {
+        double ak = ek[kp], al = el[lp];
+        double akl = ak + al;
+        double al_akl = al / akl;
+        double ckcl = cck[kp] * ccl[lp] * exp(-ak * al_akl * rr_kl);
+        double Q[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Q[d] = rk[d] + rkl[d] * al_akl;
+        for (int ip = 0; ip < iprim; ++ip)
+        for (int jp = 0; jp < jprim; ++jp) {
+            double ai = env[sp.pexpi + ip], aj = env[sp.pexpj + jp];
+            double aij = ai + aj;
+            double aj_aij = aj / aij;
+            double cicj = env[sp.pci + ip] * env[sp.pcj + jp] * exp(-ai * aj_aij * rr_ij);
+            double PQ[3], PA[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                PA[d] = sp.rij[d] * aj_aij;
+                PQ[d] = sp.ri[d] + PA[d] - Q[d];
+            }
+            double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
+            double aijkl = aij + akl;
+            double theta = aij * akl / aijkl;
+            double fac = PI_FAC * fac_sym * cicj * ckcl / (aij * akl * sqrt(aijkl));
+            double rt[N], wt[N];
+            rys_roots<N>(rys, theta * rr, rt, wt);
+#pragma unroll
+            for (int ir = 0; ir < N; ++ir) {
+                double rt_aa = rt[ir] / aijkl;
+                double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
+                double b10 = .5 / aij * (1. - rt_aij);
+                double b01 = .5 / akl * (1. - rt_akl);
+                double b00 = .5 * rt_aa;
+                double gx[GS], gy[GS], gz[GS];
+                build_g<LI, LJ, LK, LL>(gx, 1., PA[0] - rt_aij * PQ[0],
+                                        rkl[0] * al_akl + rt_akl * PQ[0],
+                                        b10, b01, b00, sp.rij[0], rkl[0]);
+                build_g<LI, LJ, LK, LL>(gy, 1., PA[1] - rt_aij * PQ[1],
+                                        rkl[1] * al_akl + rt_akl * PQ[1],
+                                        b10, b01, b00, sp.rij[1], rkl[1]);
+                build_g<LI, LJ, LK, LL>(gz, wt[ir] * fac, PA[2] - rt_aij * PQ[2],
+                                        rkl[2] * al_akl + rt_akl * PQ[2],
+                                        b10, b01, b00, sp.rij[2], rkl[2]);
+#pragma unroll
+                for (int i = 0; i < NFI; ++i)
+#pragma unroll
+                for (int j = 0; j < NFJ; ++j)
+#pragma unroll
+                for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                for (int l = 0; l < NFL; ++l) {
+                    constexpr int SJ = LL+1, SK = (LK+1)*(LL+1), SI = (LJ+1)*(LK+1)*(LL+1);
+                    int ix = ci.x[i]*SI + cj.x[j]*SK + ck.x[k]*SJ + cl.x[l];
+                    int iy = ci.y[i]*SI + cj.y[j]*SK + ck.y[k]*SJ + cl.y[l];
+                    int iz = ci.z[i]*SI + cj.z[j]*SK + ck.z[k]*SJ + cl.z[l];
+                    int n = ((i*NFJ + j)*NFK + k)*NFL + l;
+                    gout[n] = fma(gx[ix] * gy[iy], gz[iz], gout[n]);
+                }
+            }
+        }
+    }
+}
+
+template <int LI, int LJ, int LK, int LL>
+__global__ __launch_bounds__(WAVE)
+void jk_lane_kernel(JKBuildArgs a, int kl_chunk)
+{
+    constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
+    constexpr int NF = NFI * NFJ * NFK * NFL;
+    __shared__ uint32_t queue[2 * WAVE];
+    const int lane = threadIdx.x;
+    const int nbas = a.nbas, nao = a.nao;
+    WorkItem wi = work_item(a.npairs_kl, kl_chunk);
+    const uint32_t bas_ij = a.pairs_ij[wi.pij];
+    const float kl_cutoff = a.cutoff - a.q_ij[wi.pij];
+    if (a.q_kl[wi.kl0] + a.dm_penalty + Q_COND_MARGIN < kl_cutoff) return;
+    ShellPair sp;
+    load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
+    const size_t nao2 = (size_t)nao * nao;
+    int qn = 0;
+    unsigned long long nsurv = 0;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    bool stop = false;
+    for (int base = wi.kl0; base < wi.kl1 || qn > 0; base += WAVE) {
+        if (base < wi.kl1 && !stop) {
+            int pkl = base + lane;
+            bool keep = false, fail_margin = false;
+            if (pkl < wi.kl1) {
+                float q_kl = a.q_kl[pkl];
+                fail_margin = q_kl + a.dm_penalty + Q_COND_MARGIN < kl_cutoff;
+                keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, a.pairs_kl[pkl],
+                                sp.ish, sp.jsh, nbas, a.dm_cond);
+            }
+            uint64_t m = __ballot(keep);
+            if (keep) queue[qn + __popcll(m & lt_mask)] = a.pairs_kl[pkl];
+            qn += __popcll(m);
+            nsurv += __popcll(m);
+            stop = __ballot(fail_margin) != 0;
+            __syncthreads();
+        }
+        bool flush = (base + WAVE >= wi.kl1) || stop;
+        if (qn < WAVE && !flush) continue;
+        int nb = min(qn, WAVE);
+        // ---- evaluate up to 64 queued quartets, one per lane
+        double vij[NFI*NFJ];
+        bool active = lane < nb;
+        uint32_t bas_kl = queue[active ? lane : 0];
+        int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
+        double fac_sym = 1.;
+        if (sp.ish == sp.jsh) fac_sym *= .5;
+        if (ksh == lsh) fac_sym *= .5;
+        if (bas_ij == bas_kl) fac_sym *= .5;
+        if (!active) fac_sym = 0.;
+        double gout[NF];
+        eri_lane<LI, LJ, LK, LL>(gout, sp, a.iprim, a.jprim, a.kprim, a.lprim,
+                                 ksh, lsh, a.bas, a.env, a.rys, fac_sym);
+        const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
+        const int i0 = sp.i0, j0 = sp.j0;
+        for (int id = 0; id < a.n_dm; ++id) {
+            const double *D = a.dm + id * nao2;
+            if (a.vk && active) {
+                double *K = a.vk + id * nao2;
+                // K_il += (ij|kl) D_jk ; K_ik += (ij|kl) D_jl
+#pragma unroll
+                for (int i = 0; i < NFI; ++i) {
+                    double kil[NFL], kik[NFK];
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) kil[l] = 0.;
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k) kik[k] = 0.;
+#pragma unroll
+                    for (int j = 0; j < NFJ; ++j)
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) {
+                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
+                        kil[l] = fma(g, D[(j0+j)*nao + k0+k], kil[l]);
+                        kik[k] = fma(g, D[(j0+j)*nao + l0+l], kik[k]);
+                    }
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k) atomic_add(K + (i0+i)*nao + k0+k, kik[k]);
+                }
+                // K_jl += (ij|kl) D_ik ; K_jk += (ij|kl) D_il
+#pragma unroll
+                for (int j = 0; j < NFJ; ++j) {
+                    double kjl[NFL], kjk[NFK];
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) kjl[l] = 0.;
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k) kjk[k] = 0.;
+#pragma unroll
+                    for (int i = 0; i < NFI; ++i)
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) {
+                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
+                        kjl[l] = fma(g, D[(i0+i)*nao + k0+k], kjl[l]);
+                        kjk[k] = fma(g, D[(i0+i)*nao + l0+l], kjk[k]);
+                    }
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k) atomic_add(K + (j0+j)*nao + k0+k, kjk[k]);
+                }
+            }
+            if (a.vj) {
+                double *J = a.vj + id * nao2;
+                // J_kl += (ij|kl) D_ij   (per lane)
+                if (active) {
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) {
+                        double s = 0.;
+#pragma unroll
+                        for (int i = 0; i < NFI; ++i)
+#pragma unroll
+                        for (int j = 0; j < NFJ; ++j)
+                            s = fma(gout[((i*NFJ + j)*NFK + k)*NFL + l],
+                                    D[(i0+i)*nao + j0+j], s);
+                        atomic_add(J + (k0+k)*nao + l0+l, s);
+                    }
+                }
+                // J_ij += (ij|kl) D_kl   (common ij: reduce over the wave)
+#pragma unroll
+                for (int i = 0; i < NFI; ++i)
+#pragma unroll
+                for (int j = 0; j < NFJ; ++j) {
+                    double s = 0.;
+                    if (active) {
+#pragma unroll
+                        for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                        for (int l = 0; l < NFL; ++l)
+                            s = fma(gout[((i*NFJ + j)*NFK + k)*NFL + l],
+                                    D[(k0+k)*nao + l0+l], s);
+                    }
+                    vij[i*NFJ + j] = wave_sum(s);
+                }
+                if (lane == 0) {
+#pragma unroll
+                    for (int n = 0; n < NFI*NFJ; ++n)
+                        atomic_add(J + (i0 + n / NFJ)*nao + j0 + n % NFJ, vij[n]);
+                }
+            }
+        }
+        // shift the remainder of the queue to the front
+        __syncthreads();
+        uint32_t rest = (nb + lane < qn) ? queue[nb + lane] : 0u;
+        __syncthreads();
+        if (nb + lane < qn) queue[lane] = rest;
+        qn -= nb;
+        __syncthreads();
+        if (flush && qn == 0) break;
+    }
+    if (a.n_quartets && lane == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
+}
+
+// ---------------------------------------------------------------- team
+// Team of TS lanes per quartet, NQ quartets per block of TB threads.
+template <int LI, int LJ, int LK, int LL>
+struct TeamShape {
+    static constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
+    static constexpr int NROWS = NFI * NFJ;
+    static constexpr int NFKL = NFK * NFL;
+    static constexpr int N = (LI + LJ + LK + LL) / 2 + 1;
+    static constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
+    static constexpr int TS0 = NROWS > 3 * N ? NROWS : 3 * N;
+    static constexpr int TB = TS0 <= 64 ? 64 : (TS0 <= 128 ? 128 : 256);
+    static constexpr int G_DBL = 2 * N + N * 3 * GS;            // roots + 2D integrals
+    static constexpr int S_ROWS0 = 4096 / NFKL;                 // rows staged per pass
+    static constexpr int S_ROWS = S_ROWS0 < NROWS ? (S_ROWS0 > 0 ? S_ROWS0 : 1) : NROWS;
+    static constexpr int S_DBL = S_ROWS * NFKL;
+    static constexpr int PER_Q = G_DBL > S_DBL ? G_DBL : S_DBL;
+    static constexpr int NQ0 = TB / TS0;
+    static constexpr int NQ1 = (6144 / PER_Q) > 0 ? (6144 / PER_Q) : 1;   // 48 KB of LDS
+    static constexpr int NQ = NQ0 < NQ1 ? NQ0 : NQ1;
+    static constexpr int TS = TB / NQ;
+};
+
+template <int LI, int LJ, int LK, int LL>
+__global__ __launch_bounds__((TeamShape<LI, LJ, LK, LL>::TB))
+void jk_team_kernel(JKBuildArgs a, int kl_chunk)
+{
+    using S = TeamShape<LI, LJ, LK, LL>;
+    constexpr int NFI = S::NFI, NFJ = S::NFJ, NFK = S::NFK, NFL = S::NFL;
+    constexpr int NROWS = S::NROWS, NFKL = S::NFKL, N = S::N, GS = S::GS;
+    constexpr int TB = S::TB, NQ = S::NQ, TS = S::TS;
+    constexpr int NW = TB / WAVE;
+    constexpr CartPow<LI> ci{};
+    constexpr CartPow<LJ> cj{};
+    constexpr CartPow<LK> ck{};
+    constexpr CartPow<LL> cl{};
+    constexpr int SJ = LL+1, SK = (LK+1)*(LL+1), SI = (LJ+1)*(LK+1)*(LL+1);
+
+    __shared__ uint32_t queue[2 * TB];
+    __shared__ int wave_cnt[NW + 1];
+    __shared__ int stop_flag;
+    __shared__ double lds[NQ * S::PER_Q];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wid = tid / WAVE;
+    const int team = tid / TS;
+    const int tl = tid - team * TS;
+    const int nbas = a.nbas, nao = a.nao;
+    const size_t nao2 = (size_t)nao * nao;
+    WorkItem wi = work_item(a.npairs_kl, kl_chunk);
+    const uint32_t bas_ij = a.pairs_ij[wi.pij];
+    const float kl_cutoff = a.cutoff - a.q_ij[wi.pij];
+    if (a.q_kl[wi.kl0] + a.dm_penalty + Q_COND_MARGIN < kl_cutoff) return;
+    ShellPair sp;
+    load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    double *my = lds + (team < NQ ? team : 0) * S::PER_Q;
+    double *rw = my;                  // [2N]
+    double *gbuf = my + 2 * N;        // [N][3][GS]
+    const int row = tl;               // (i, j) row owned by this lane
+    const bool has_row = team < NQ && row < NROWS;
+    const int irow = has_row ? row / NFJ : 0, jrow = has_row ? row % NFJ : 0;
+    double rr_ij = sp.rij[0]*sp.rij[0] + sp.rij[1]*sp.rij[1] + sp.rij[2]*sp.rij[2];
+
+    int qn = 0;
+    unsigned long long nsurv = 0;
+    bool stop = false;
+    for (int base = wi.kl0; ; base += TB) {
+        if (base < wi.kl1 && !stop) {
+            int pkl = base + tid;
+            bool keep = false, fail_margin = false;
+            uint32_t bkl = 0;
+            if (pkl < wi.kl1) {
+                float q_kl = a.q_kl[pkl];
+                bkl = a.pairs_kl[pkl];
+                fail_margin = q_kl + a.dm_penalty + Q_COND_MARGIN < kl_cutoff;
+                keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
+                                sp.ish, sp.jsh, nbas, a.dm_cond);
+            }
+            uint64_t m = __ballot(keep);
+            uint64_t mf = __ballot(fail_margin);
+            if (tid == 0) stop_flag = 0;
+            __syncthreads();
+            if (lane == 0) {
+                wave_cnt[wid] = __popcll(m);
+                if (mf) stop_flag = 1;
+            }
+            __syncthreads();
+            int off = qn;
+            for (int w = 0; w < wid; ++w) off += wave_cnt[w];
+            if (keep) queue[off + __popcll(m & lt_mask)] = bkl;
+            int tot = 0;
+            for (int w = 0; w < NW; ++w) tot += wave_cnt[w];
+            qn += tot;
+            nsurv += tot;
+            stop = stop_flag != 0;
+            __syncthreads();
+        }
+        bool flush = base + TB >= wi.kl1 || stop;
+        int head = 0;
+        while (qn - head >= NQ || (flush && head < qn)) {
+            // team `team` evaluates queue[head + team]
+            const int qidx = head + team;
+            const bool active = team < NQ && qidx < qn;
+            uint32_t bas_kl = queue[active ? qidx : head];
+            int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
+            double fac_sym = 1.;
+            if (sp.ish == sp.jsh) fac_sym *= .5;
+            if (ksh == lsh) fac_sym *= .5;
+            if (bas_ij == bas_kl) fac_sym *= .5;
+            const int32_t *bk = a.bas + ksh * 8, *bl = a.bas + lsh * 8;
+            const double *rk = a.env + bk[7], *rl = a.env + bl[7];
+            double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
+            double rr_kl = rkl[0]*rkl[0] + rkl[1]*rkl[1] + rkl[2]*rkl[2];
+            const double *ek = a.env + bk[5], *el = a.env + bl[5];
+            const double *cck = a.env + bk[6], *ccl = a.env + bl[6];
+            double acc[NFKL];
+#pragma unroll
+            for (int n = 0; n < NFKL; ++n) acc[n] = 0.;
+            for (int kp = 0; kp < a.kprim; ++kp)
+            for (int lp = 0; lp < a.lprim; ++lp) {
+                double ak = ek[kp], al = el[lp];
+                double akl = ak + al;
+                double al_akl = al / akl;
+                double ckcl = cck[kp] * ccl[lp] * exp(-ak * al_akl * rr_kl);
+                double Q[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) Q[d] = rk[d] + rkl[d] * al_akl;
+                for (int ip = 0; ip < a.iprim; ++ip)
+                for (int jp = 0; jp < a.jprim; ++jp) {
+                    double ai = a.env[sp.pexpi + ip], aj = a.env[sp.pexpj + jp];
+                    double aij = ai + aj;
+                    double aj_aij = aj / aij;
+                    double cicj = a.env[sp.pci + ip] * a.env[sp.pcj + jp]
+                                * exp(-ai * aj_aij * rr_ij);
+                    double PQ[3], PA[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        PA[d] = sp.rij[d] * aj_aij;
+                        PQ[d] = sp.ri[d] + PA[d] - Q[d];
+                    }
+                    double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
+                    double aijkl = aij + akl;
+                    double theta = aij * akl / aijkl;
+                    double fac = PI_FAC * fac_sym * cicj * ckcl / (aij * akl * sqrt(aijkl));
+                    __syncthreads();
+                    if (active && tl == 0) {
+                        double rt[N], wt[N];
+                        rys_roots<N>(a.rys, theta * rr, rt, wt);
+#pragma unroll
+                        for (int r = 0; r < N; ++r) { rw[2*r] = rt[r]; rw[2*r+1] = wt[r]; }
+                    }
+                    __syncthreads();
+                    if (active && tl < 3 * N) {
+                        int r = tl / 3, d = tl - (tl / 3) * 3;
+                        double rt_aa = rw[2*r] / aijkl;
+                        double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
+                        double b10 = .5 / aij * (1. - rt_aij);
+                        double b01 = .5 / akl * (1. - rt_akl);
+                        double b00 = .5 * rt_aa;
+                        double g[GS];
+                        double pa = d == 0 ? PA[0] : (d == 1 ? PA[1] : PA[2]);
+                        double pq = d == 0 ? PQ[0] : (d == 1 ? PQ[1] : PQ[2]);
+                        double rd = d == 0 ? sp.rij[0] : (d == 1 ? sp.rij[1] : sp.rij[2]);
+                        double kd = d == 0 ? rkl[0] : (d == 1 ? rkl[1] : rkl[2]);
+                        double g00 = d == 2 ? rw[2*r+1] * fac : 1.;
+                        build_g<LI, LJ, LK, LL>(g, g00, pa - rt_aij * pq,
+                                                kd * al_akl + rt_akl * pq,
+                                                b10, b01, b00, rd, kd);
+                        double *dst = gbuf + (r * 3 + d) * GS;
+#pragma unroll
+                        for (int n = 0; n < GS; ++n) dst[n] = g[n];
+                    }
+                    __syncthreads();
+                    if (active && has_row) {
+                        int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+                        // Cartesian powers of this row's i and j (runtime lookup)
+#pragma unroll
+                        for (int i = 0; i < NFI; ++i)
+                            if (i == irow) { ax = ci.x[i]; ay = ci.y[i]; az = ci.z[i]; }
+#pragma unroll
+                        for (int j = 0; j < NFJ; ++j)
+                            if (j == jrow) { bx = cj.x[j]; by = cj.y[j]; bz = cj.z[j]; }
+                        const int ox = ax * SI + bx * SK, oy = ay * SI + by * SK, oz = az * SI + bz * SK;
+                        for (int r = 0; r < N; ++r) {
+                            const double *gx = gbuf + (r * 3 + 0) * GS + ox;
+                            const double *gy = gbuf + (r * 3 + 1) * GS + oy;
+                            const double *gz = gbuf + (r * 3 + 2) * GS + oz;
+                            double sx[SK], sy[SK], sz[SK];
+#pragma unroll
+                            for (int n = 0; n < SK; ++n) { sx[n] = gx[n]; sy[n] = gy[n]; sz[n] = gz[n]; }
+#pragma unroll
+                            for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                            for (int l = 0; l < NFL; ++l) {
+                                int kx = ck.x[k]*SJ + cl.x[l];
+                                int ky = ck.y[k]*SJ + cl.y[l];
+                                int kz = ck.z[k]*SJ + cl.z[l];
+                                acc[k*NFL + l] = fma(sx[kx] * sy[ky], sz[kz], acc[k*NFL + l]);
+                            }
+                        }
+                    }
+                }
+            }
+            // ---- digestion of the team's quartet
+            const int i0 = sp.i0, j0 = sp.j0;
+            const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
+            double *stage = my;       // reuse the team's LDS
+            for (int id = 0; id < a.n_dm; ++id) {
+                const double *D = a.dm + id * nao2;
+                if (a.vj && active && has_row) {
+                    double s = 0.;
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l)
+                        s = fma(acc[k*NFL + l], D[(k0+k)*nao + l0+l], s);
+                    atomic_add(a.vj + id * nao2 + (i0+irow)*nao + j0+jrow, s);
+                }
+                for (int r0 = 0; r0 < NROWS; r0 += S::S_ROWS) {
+                    const int r1 = min(r0 + S::S_ROWS, NROWS);
+                    __syncthreads();
+                    if (active && has_row && row >= r0 && row < r1) {
+#pragma unroll
+                        for (int n = 0; n < NFKL; ++n) stage[(row - r0) * NFKL + n] = acc[n];
+                    }
+                    __syncthreads();
+                    if (!active) continue;
+                    if (a.vj) {
+                        double *J = a.vj + id * nao2;
+                        for (int e = tl; e < NFKL; e += TS) {
+                            int k = e / NFL, l = e - (e / NFL) * NFL;
+                            double s = 0.;
+                            for (int rr = r0; rr < r1; ++rr)
+                                s = fma(stage[(rr - r0) * NFKL + e],
+                                        D[(i0 + rr / NFJ)*nao + j0 + rr % NFJ], s);
+                            atomic_add(J + (k0+k)*nao + l0+l, s);
+                        }
+                    }
+                    if (a.vk) {
+                        double *K = a.vk + id * nao2;
+                        const int ilo = r0 / NFJ, ihi = (r1 - 1) / NFJ;
+                        const int ni = ihi - ilo + 1;
+                        // K_ik (sum over j,l of D_jl) and K_il (sum over j,k of D_jk)
+                        for (int e = tl; e < ni * (NFK + NFL); e += TS) {
+                            int i = ilo + e / (NFK + NFL);
+                            int m = e - (e / (NFK + NFL)) * (NFK + NFL);
+                            double s = 0.;
+                            for (int j = 0; j < NFJ; ++j) {
+                                int rr = i * NFJ + j;
+                                if (rr < r0 || rr >= r1) continue;
+                                const double *srow = stage + (rr - r0) * NFKL;
+                                if (m < NFK) {
+                                    for (int l = 0; l < NFL; ++l)
+                                        s = fma(srow[m*NFL + l], D[(j0+j)*nao + l0+l], s);
+                                } else {
+                                    int l = m - NFK;
+                                    for (int k = 0; k < NFK; ++k)
+                                        s = fma(srow[k*NFL + l], D[(j0+j)*nao + k0+k], s);
+                                }
+                            }
+                            if (m < NFK) atomic_add(K + (i0+i)*nao + k0+m, s);
+                            else atomic_add(K + (i0+i)*nao + l0+m-NFK, s);
+                        }
+                        // K_jk (sum over i,l of D_il) and K_jl (sum over i,k of D_ik)
+                        for (int e = tl; e < NFJ * (NFK + NFL); e += TS) {
+                            int j = e / (NFK + NFL);
+                            int m = e - j * (NFK + NFL);
+                            double s = 0.;
+                            for (int i = ilo; i <= ihi; ++i) {
+                                int rr = i * NFJ + j;
+                                if (rr < r0 || rr >= r1) continue;
+                                const double *srow = stage + (rr - r0) * NFKL;
+                                if (m < NFK) {
+                                    for (int l = 0; l < NFL; ++l)
+                                        s = fma(srow[m*NFL + l], D[(i0+i)*nao + l0+l], s);
+                                } else {
+                                    int l = m - NFK;
+                                    for (int k = 0; k < NFK; ++k)
+                                        s = fma(srow[k*NFL + l], D[(i0+i)*nao + k0+k], s);
+                                }
+                            }
+                            if (m < NFK) atomic_add(K + (j0+j)*nao + k0+m, s);
+                            else atomic_add(K + (j0+j)*nao + l0+m-NFK, s);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            head += NQ;
+        }
+        // move the unprocessed remainder to the front of the queue
+        int rem = qn - head;
+        uint32_t v = 0;
+        if (tid < rem) v = queue[head + tid];
+        __syncthreads();
+        if (tid < rem) queue[tid] = v;
+        qn = rem;
+        __syncthreads();
+        if (flush) break;
+    }
+    if (a.n_quartets && tid == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
+}
+
+// ---------------------------------------------------------------- dispatch
+template <int LI, int LJ, int LK, int LL>
+struct ClassTraits {
+    static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
+    static constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
+    // per-lane path when the block and three 2D tables fit in ~120 doubles
+    static constexpr bool LANE = NF + 3 * GS <= 120;
+};
+
+template <int LI, int LJ, int LK, int LL>
+static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
+{
+    if constexpr (ClassTraits<LI, LJ, LK, LL>::LANE) {
+        const int kl_chunk = 1024;
+        int nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
+        long nblocks = (long)a.npairs_ij * nchunks;
+        if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+        hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(WAVE), 0,
+                           stream, a, kl_chunk);
+    } else {
+        using S = TeamShape<LI, LJ, LK, LL>;
+        const int kl_chunk = 256;
+        int nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
+        long nblocks = (long)a.npairs_ij * nchunks;
+        if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+        hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB), 0,
+                           stream, a, kl_chunk);
+    }
+    return hipGetLastError();
+}
+
+// Angular classes reachable from groups sorted by l: li >= lj, li >= lk >= ll.
+#define JK_CASE(I, J, K, L) \
+    case ((I)*125 + (J)*25 + (K)*5 + (L)): return launch_class<I, J, K, L>(a, stream);
+
+#include "jk_dispatch.inc"
+
+static hipError_t dispatch(const JKBuildArgs &a, hipStream_t stream)
+{
+    int key = a.li * 125 + a.lj * 25 + a.lk * 5 + a.ll;
+    return dispatch_class(key, a, stream);
+}
+
+// ---------------------------------------------------------------- setup kernels
+__global__ void ovlp_mask_kernel(int8_t *mask, const float *exps, const float *log_coeff,
+                                 const double *coords, const int32_t *ls, int nbas,
+                                 float log_cutoff)
+{
+    int jsh = blockIdx.x * blockDim.x + threadIdx.x;
+    int ish = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ish >= nbas || jsh >= nbas) return;
+    int8_t keep = 0;
+    if (ish >= jsh) {
+        // lib/pbc/overlap.cu:2163-2223, float32 in the same order
+        float xi = coords[ish*3+0], yi = coords[ish*3+1], zi = coords[ish*3+2];
+        float xj = coords[jsh*3+0], yj = coords[jsh*3+1], zj = coords[jsh*3+2];
+        float ai = exps[ish], aj = exps[jsh];
+        float aij = ai + aj;
+        float fi = ai / aij, fj = aj / aij;
+        float theta = ai * fj;
+        float dx = xj - xi, dy = yj - yi, dz = zj - zi;
+        float rr = dx * dx + dy * dy + dz * dz;
+        float li = ls[ish], lj = ls[jsh];
+        if (!(theta * rr > 50.f)) {
+            float fac_norm = log_coeff[ish] + log_coeff[jsh] + 1.717f - 1.5f * logf(aij);
+            float dr = sqrtf(rr);
+            float dri = fj * dr, drj = fi * dr;
+            float dri_fac = .5f * li * logf(.5f * li / aij + dri * dri + 1e-9f);
+            float drj_fac = .5f * lj * logf(.5f * lj / aij + drj * drj + 1e-9f);
+            float log_ovlp = fac_norm - theta * rr + dri_fac + drj_fac;
+            keep = log_ovlp > log_cutoff;
+        }
+    }
+    mask[(size_t)ish * nbas + jsh] = keep;
+}
+
+// max over the Cartesian diagonal (ab|ab) of a shell pair; by the
+// Cauchy-Schwarz inequality this is max |(ij|ij)| over the whole block.
+__global__ void pair_qcond_kernel(float *q, const uint32_t *pairs, int npairs,
+                                  const int32_t *bas, const double *env, int nbas,
+                                  JKRysTable rys)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    uint32_t bij = pairs[p];
+    int ish = bij / nbas, jsh = bij - (bij / nbas) * nbas;
+    const int32_t *bi = bas + ish * 8, *bj = bas + jsh * 8;
+    int li = bi[1], lj = bj[1];
+    int lij = li + lj, n = lij + 1;
+    int nfi = ncart(li), nfj = ncart(lj);
+    const double *ri = env + bi[7], *rj = env + bj[7];
+    double rij[3] = {rj[0] - ri[0], rj[1] - ri[1], rj[2] - ri[2]};
+    double rr_ij = rij[0]*rij[0] + rij[1]*rij[1] + rij[2]*rij[2];
+    // gd[d][a][b] = 2D integral (a b | a b) in direction d, accumulated per root
+    double out[15*15];
+    for (int t = 0; t < nfi * nfj; ++t) out[t] = 0.;
+    int px[15], py[15], pz[15], qx[15], qy[15], qz[15];
+    {
+        int c = 0;
+        for (int x = li; x >= 0; --x) for (int y = li - x; y >= 0; --y) { px[c] = x; py[c] = y; pz[c] = li-x-y; ++c; }
+        c = 0;
+        for (int x = lj; x >= 0; --x) for (int y = lj - x; y >= 0; --y) { qx[c] = x; qy[c] = y; qz[c] = lj-x-y; ++c; }
+    }
+    for (int ip = 0; ip < bi[2]; ++ip)
+    for (int jp = 0; jp < bj[2]; ++jp)
+    for (int kp = 0; kp < bi[2]; ++kp)
+    for (int lp = 0; lp < bj[2]; ++lp) {
+        double ai = env[bi[5]+ip], aj = env[bj[5]+jp];
+        double ak = env[bi[5]+kp], al = env[bj[5]+lp];
+        double aij = ai + aj, akl = ak + al;
+        double cicj = env[bi[6]+ip] * env[bj[6]+jp] * exp(-ai * aj / aij * rr_ij);
+        double ckcl = env[bi[6]+kp] * env[bj[6]+lp] * exp(-ak * al / akl * rr_ij);
+        double PQ[3], PA[3], QC[3];
+        for (int d = 0; d < 3; ++d) {
+            PA[d] = rij[d] * aj / aij;
+            QC[d] = rij[d] * al / akl;
+            PQ[d] = PA[d] - QC[d];
+        }
+        double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
+        double aijkl = aij + akl;
+        double theta = aij * akl / aijkl;
+        double fac = PI_FAC * cicj * ckcl / (aij * akl * sqrt(aijkl));
+        double rt[JK_RYS_NMAX], wt[JK_RYS_NMAX];
+        rys_roots_rt(rys, n, theta * rr, rt, wt);
+        for (int ir = 0; ir < n; ++ir) {
+            double rt_aa = rt[ir] / aijkl;
+            double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
+            double b10 = .5 / aij * (1. - rt_aij);
+            double b01 = .5 / akl * (1. - rt_akl);
+            double b00 = .5 * rt_aa;
+            double gd[3][5][5];
+            for (int d = 0; d < 3; ++d) {
+                double G[9][9];
+                double c0 = PA[d] - rt_aij * PQ[d];
+                double cp = QC[d] + rt_akl * PQ[d];
+                G[0][0] = d == 2 ? wt[ir] * fac : 1.;
+                if (lij > 0) G[1][0] = c0 * G[0][0];
+                for (int a2 = 1; a2 < lij; ++a2) G[a2+1][0] = c0 * G[a2][0] + a2 * b10 * G[a2-1][0];
+                for (int a2 = 0; a2 <= lij; ++a2) {
+                    if (lij > 0) {
+                        double v = cp * G[a2][0];
+                        if (a2 > 0) v += a2 * b00 * G[a2-1][0];
+                        G[a2][1] = v;
+                    }
+                    for (int c = 1; c < lij; ++c) {
+                        double v = cp * G[a2][c] + c * b01 * G[a2][c-1];
+                        if (a2 > 0) v += a2 * b00 * G[a2-1][c];
+                        G[a2][c+1] = v;
+                    }
+                }
+                // bra HRR for all (a, b, c); then ket HRR only for (c, e) = (a, b)
+                double H[9][5][9];
+                for (int c = 0; c <= lij; ++c) {
+                    for (int a2 = 0; a2 <= lij; ++a2) H[a2][0][c] = G[a2][c];
+                    for (int b = 0; b < lj; ++b)
+                        for (int a2 = 0; a2 < lij - b; ++a2)
+                            H[a2][b+1][c] = H[a2+1][b][c] - rij[d] * H[a2][b][c];
+                }
+                for (int a2 = 0; a2 <= li; ++a2)
+                for (int b = 0; b <= lj; ++b) {
+                    double T[9][5];
+                    for (int c = 0; c <= lij; ++c) T[c][0] = H[a2][b][c];
+                    for (int e = 0; e < lj; ++e)
+                        for (int c = 0; c < lij - e; ++c)
+                            T[c][e+1] = T[c+1][e] - rij[d] * T[c][e];
+                    gd[d][a2][b] = T[a2][b];
+                }
+            }
+            for (int i = 0; i < nfi; ++i)
+            for (int j = 0; j < nfj; ++j)
+                out[i*nfj+j] += gd[0][px[i]][qx[j]] * gd[1][py[i]][qy[j]] * gd[2][pz[i]][qz[j]];
+        }
+    }
+    double m = 0.;
+    for (int t = 0; t < nfi * nfj; ++t) m = fmax(m, fabs(out[t]));
+    q[p] = m > 0 ? (float)(.5 * log(m)) : -700.f;
+}
+
+__global__ void dm_block_max_kernel(double *work, const double *dm, int n_dm, int nao,
+                                    const int32_t *ao_loc, int nbas)
+{
+    int jsh = blockIdx.x * blockDim.x + threadIdx.x;
+    int ish = blockIdx.y;
+    if (jsh >= nbas) return;
+    int i0 = ao_loc[ish], i1 = ao_loc[ish+1], j0 = ao_loc[jsh], j1 = ao_loc[jsh+1];
+    double m = 0.;
+    for (int id = 0; id < n_dm; ++id) {
+        const double *D = dm + (size_t)id * nao * nao;
+        for (int i = i0; i < i1; ++i)
+            for (int j = j0; j < j1; ++j)
+                m = fmax(m, fabs(D[(size_t)i * nao + j]));
+    }
+    work[(size_t)ish * nbas + jsh] = m;
+}
+
+__global__ void dm_cond_log_kernel(float *cond, const double *work, int nbas, int hermi)
+{
+    int jsh = blockIdx.x * blockDim.x + threadIdx.x;
+    int ish = blockIdx.y;
+    if (jsh >= nbas) return;
+    double v = work[(size_t)ish * nbas + jsh];
+    if (hermi == 0) v = v + work[(size_t)jsh * nbas + ish];
+    cond[(size_t)ish * nbas + jsh] = (float)log(v + 1e-300);
+}
+
+// ---------------------------------------------------------------- C-ABI
+static bool table_ok(const JKRysTable &t)
+{
+    return t.coef != nullptr && t.degree > 0 && t.degree <= RYS_DEG_MAX;
+}
+
+extern "C" {
+
+const char *jk_last_error(void) { return g_err; }
+
+int jk_abi_version(void) { return JK_ABI_VERSION; }
+
+int jk_build(const JKBuildArgs *args, void *stream)
+{
+    g_err[0] = 0;
+    if (!args) { set_err("jk_build: null args"); return 1; }
+    const JKBuildArgs &a = *args;
+    if (a.abi_version != JK_ABI_VERSION) {
+        set_err("jk_build: ABI version %s%d != %d", "", a.abi_version, JK_ABI_VERSION);
+        return 1;
+    }
+    if (a.li < 0 || a.li > JK_LMAX || a.lj < 0 || a.lj > a.li || a.lk < 0 ||
+        a.lk > a.li || a.ll < 0 || a.ll > a.lk) {
+        set_err("jk_build: unsupported angular class (%s li=%d, lj=%d...)", "", a.li, a.lj);
+        return 2;
+    }
+    if (a.npairs_ij <= 0 || a.npairs_kl <= 0) return 0;
+    if (!a.bas || !a.env || !a.ao_loc || !a.pairs_ij || !a.pairs_kl || !a.q_ij ||
+        !a.q_kl || !a.dm_cond || !a.dm || (!a.vj && !a.vk) || a.n_dm <= 0 ||
+        a.nbas <= 0 || a.nao <= 0) {
+        set_err("jk_build: null pointer or empty dimension");
+        return 3;
+    }
+    if (!table_ok(a.rys)) { set_err("jk_build: Rys table not set"); return 3; }
+    int nroots = (a.li + a.lj + a.lk + a.ll) / 2 + 1;
+    if (nroots > JK_RYS_NMAX || a.rys.xmax[nroots-1] <= 0) {
+        set_err("jk_build: Rys table lacks %s%d roots", "", nroots);
+        return 3;
+    }
+    hipError_t err = dispatch(a, (hipStream_t)stream);
+    if (err != hipSuccess) {
+        snprintf(g_err, sizeof(g_err), "jk_build (%d%d|%d%d): %s", a.li, a.lj, a.lk, a.ll,
+                 hipGetErrorString(err));
+        return 4;
+    }
+    return 0;
+}
+
+int jk_ovlp_mask(int8_t *mask, const float *exps, const float *log_coeff,
+                 const double *coords, const int32_t *ls, int nbas,
+                 float log_cutoff, void *stream)
+{
+    g_err[0] = 0;
+    if (!mask || !exps || !log_coeff || !coords || !ls || nbas <= 0) {
+        set_err("jk_ovlp_mask: bad arguments"); return 3;
+    }
+    dim3 t(16, 16), b((nbas + 15) / 16, (nbas + 15) / 16);
+    hipLaunchKernelGGL(ovlp_mask_kernel, b, t, 0, (hipStream_t)stream,
+                       mask, exps, log_coeff, coords, ls, nbas, log_cutoff);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) { set_err("jk_ovlp_mask: %s", hipGetErrorString(err)); return 4; }
+    return 0;
+}
+
+int jk_pair_qcond(float *q, const uint32_t *pairs, int npairs,
+                  const int32_t *bas, const double *env, int nbas,
+                  JKRysTable rys, void *stream)
+{
+    g_err[0] = 0;
+    if (npairs <= 0) return 0;
+    if (!q || !pairs || !bas || !env || nbas <= 0 || !table_ok(rys)) {
+        set_err("jk_pair_qcond: bad arguments"); return 3;
+    }
+    hipLaunchKernelGGL(pair_qcond_kernel, dim3((npairs + 63) / 64), dim3(64), 0,
+                       (hipStream_t)stream, q, pairs, npairs, bas, env, nbas, rys);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) { set_err("jk_pair_qcond: %s", hipGetErrorString(err)); return 4; }
+    return 0;
+}
+
+int jk_dm_cond(float *dm_cond, double *work, const double *dm, int n_dm,
+               int nao, const int32_t *ao_loc, int nbas, int hermi, void *stream)
+{
+    g_err[0] = 0;
+    if (!dm_cond || !work || !dm || !ao_loc || n_dm <= 0 || nao <= 0 || nbas <= 0) {
+        set_err("jk_dm_cond: bad arguments"); return 3;
+    }
+    dim3 b((nbas + 63) / 64, nbas);
+    hipLaunchKernelGGL(dm_block_max_kernel, b, dim3(64), 0, (hipStream_t)stream,
+                       work, dm, n_dm, nao, ao_loc, nbas);
+    hipLaunchKernelGGL(dm_cond_log_kernel, b, dim3(64), 0, (hipStream_t)stream,
+                       dm_cond, work, nbas, hermi);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) { set_err("jk_dm_cond: %s", hipGetErrorString(err)); return 4; }
+    return 0;
+}
+
+}  // extern "C"

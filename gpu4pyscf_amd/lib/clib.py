@@ -1,0 +1,125 @@
+'''ctypes binding of libjkhip.so (include/jkhip.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C gpu4pyscf_amd/csrc``).  There is no CPU fallback: if the library
+is missing, importing the J/K engine fails with a RuntimeError.
+'''
+
+import ctypes
+import os
+
+import numpy as np
+
+_here = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_here, 'libjkhip.so')
+TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
+
+JK_ABI_VERSION = 1
+JK_RYS_NMAX = 11
+JK_LMAX = 3
+
+c_void_p = ctypes.c_void_p
+
+
+class JKRysTable(ctypes.Structure):
+    _fields_ = [
+        ('coef', c_void_p),
+        ('off', ctypes.c_int64 * JK_RYS_NMAX),
+        ('xmax', ctypes.c_double * JK_RYS_NMAX),
+        ('large_r', ctypes.c_int64),
+        ('large_w', ctypes.c_int64),
+        ('degree', ctypes.c_int),
+    ]
+
+
+class JKBuildArgs(ctypes.Structure):
+    _fields_ = [
+        ('abi_version', ctypes.c_int),
+        ('nbas', ctypes.c_int),
+        ('nao', ctypes.c_int),
+        ('bas', c_void_p),
+        ('env', c_void_p),
+        ('ao_loc', c_void_p),
+        ('li', ctypes.c_int), ('lj', ctypes.c_int), ('lk', ctypes.c_int), ('ll', ctypes.c_int),
+        ('iprim', ctypes.c_int), ('jprim', ctypes.c_int),
+        ('kprim', ctypes.c_int), ('lprim', ctypes.c_int),
+        ('npairs_ij', ctypes.c_int), ('npairs_kl', ctypes.c_int),
+        ('pairs_ij', c_void_p), ('pairs_kl', c_void_p),
+        ('q_ij', c_void_p), ('q_kl', c_void_p),
+        ('dm_cond', c_void_p),
+        ('cutoff', ctypes.c_float),
+        ('dm_penalty', ctypes.c_float),
+        ('n_dm', ctypes.c_int),
+        ('dm', c_void_p),
+        ('vj', c_void_p),
+        ('vk', c_void_p),
+        ('rys', JKRysTable),
+        ('n_quartets', c_void_p),
+    ]
+
+
+EXPORTS = ('jk_build', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
+           'jk_last_error', 'jk_abi_version')
+
+_lib = None
+
+
+def load():
+    '''Load libjkhip.so; raises RuntimeError when it has not been built.'''
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f'{LIB_PATH} is not built; run __graft_entry__.build() '
+                           'or make -C gpu4pyscf_amd/csrc')
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.jk_last_error.restype = ctypes.c_char_p
+    for name in ('jk_build', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond', 'jk_abi_version'):
+        getattr(lib, name).restype = ctypes.c_int
+    lib.jk_build.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p]
+    lib.jk_pair_qcond.argtypes = [c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p,
+                                  ctypes.c_int, JKRysTable, c_void_p]
+    if lib.jk_abi_version() != JK_ABI_VERSION:
+        raise RuntimeError('libjkhip.so ABI version mismatch; rebuild it')
+    _lib = lib
+    return lib
+
+
+def check(status, what):
+    if status != 0:
+        msg = load().jk_last_error().decode()
+        raise RuntimeError(f'{what} failed (status {status}): {msg}')
+
+
+def load_rys_table_host():
+    t = np.load(TABLE_PATH)
+    coef = t['coef']
+    lr = t['large_r']
+    lw = t['large_w']
+    data = np.concatenate([coef, lr, lw]).astype(np.float64)
+    meta = dict(off=t['off'].astype(np.int64), xmax=t['xmax'].astype(np.float64),
+                large_r=len(coef), large_w=len(coef) + len(lr), degree=int(t['degree']))
+    return data, meta
+
+
+_tables = {}
+
+
+def rys_table(device):
+    '''(device tensor, JKRysTable) of the Rys Chebyshev table on ``device``.'''
+    import torch
+    key = str(device)
+    if key not in _tables:
+        data, meta = load_rys_table_host()
+        buf = torch.as_tensor(data, device=device)
+        tab = JKRysTable()
+        tab.coef = buf.data_ptr()
+        n = len(meta['off'])
+        for i in range(JK_RYS_NMAX):
+            tab.off[i] = int(meta['off'][i]) if i < n else 0
+            tab.xmax[i] = float(meta['xmax'][i]) if i < n else 0.
+        tab.large_r = meta['large_r']
+        tab.large_w = meta['large_w']
+        tab.degree = meta['degree']
+        _tables[key] = (buf, tab)
+    return _tables[key]

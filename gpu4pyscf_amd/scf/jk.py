@@ -1,0 +1,336 @@
+'''Direct-SCF Coulomb (J) and exchange (K) matrices on MI355X.
+
+Drop-in for gpu4pyscf.scf.jk (gpu4pyscf/scf/jk.py:60-230, 407-641): the same
+``get_jk``/``get_j``/``get_k`` signatures and a ``_VHFOpt`` screening object
+with the same attribute surface (``sorted_mol``, ``bas_pair_cache``,
+``h_shls``, ``coeff``, ``apply_coeff_C_mat_CT``, ``apply_coeff_CT_mat_C``,
+``get_jk(dms, hermi, verbose)``).  Device buffers are torch tensors; every
+integral is evaluated by the HIP kernels of libjkhip.so.
+
+Accepted ``dm``: numpy arrays (returned as numpy) or torch tensors (returned
+as torch tensors on the same device).  omega != 0 raises NotImplementedError;
+shells with l > 3 raise NotImplementedError (the reference sends l > 4 to
+PySCF on the CPU, gpu4pyscf/scf/jk.py:622-640; no CPU path exists here).
+'''
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+from gpu4pyscf_amd.lib import clib
+
+__all__ = ['get_jk', 'get_j', 'get_k', '_VHFOpt']
+
+TILE = 12                 # pair enumeration tile, gpu4pyscf/scf/jk.py:51
+Q_COND_MARGIN = 4.        # bin width of the descending q_cond ordering
+LMAX = clib.JK_LMAX
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _device():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError('the J/K engine needs a HIP device (MI355X); none is visible')
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+def _stream_ptr():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def tile_pairs(ish0, ish1, jsh0, jsh1, nbas, tile=TILE):
+    '''Pair ids ish*nbas+jsh enumerated tile by tile (row-major inside a
+    tile), the order of gpu4pyscf/lib/pbc/supmol_sr_estimator.cu:461-487.'''
+    ni, nj = ish1 - ish0, jsh1 - jsh0
+    if ni == 0 or nj == 0:
+        return np.zeros(0, dtype=np.int64)
+    ti = np.arange(ni) // tile
+    tj = np.arange(nj) // tile
+    ii, jj = np.meshgrid(np.arange(ni), np.arange(nj), indexing='ij')
+    key = (ti[ii] * (nj // tile + 1) + tj[jj]) * (tile * tile) + (ii % tile) * tile + jj % tile
+    order = np.argsort(key.ravel(), kind='stable')
+    pairs = (ii.ravel() + ish0) * nbas + (jj.ravel() + jsh0)
+    return pairs[order]
+
+
+def group_by_split_points(q, split_points):
+    '''Indices of ``q`` in descending 4-wide log bins; bin 0 (q <= log tol)
+    is dropped (gpu4pyscf/pbc/scf/rsjk.py:2268-2277).'''
+    bins = np.searchsorted(split_points, q.astype(np.float64))
+    nb = len(split_points)
+    subsets = [np.where(bins == i)[0] for i in range(nb, 0, -1)]
+    if not subsets:
+        return np.zeros(0, dtype=np.int64)
+    return np.concatenate(subsets).astype(np.int64)
+
+
+class _VHFOpt:
+    '''Screening object of the direct J/K build (gpu4pyscf/scf/jk.py:407-486).'''
+
+    def __init__(self, mol, direct_scf_tol=1e-13, tile=TILE):
+        self.mol = mol
+        self.direct_scf_tol = direct_scf_tol
+        self.tile = tile
+        self.h_shls = None
+        self.sorted_mol = None
+        self.bas_pair_cache = None
+        self._dev = None
+        self.timings = {}
+        self.last_quartets = None
+
+    def reset(self, mol):
+        self.mol = mol
+        self.sorted_mol = None
+        self.bas_pair_cache = None
+        self._dev = None
+
+    # -- setup -------------------------------------------------------------
+    def build(self, group_size=None, verbose=None):
+        torch = _torch()
+        t0 = time.perf_counter()
+        lib = clib.load()
+        dev = _device()
+        mol = self.sorted_mol = SortedGTO(self.mol, diffuse_cutoff=0.3)
+        lmax = int(mol.uniq_l_ctr[:, 0].max())
+        if lmax > LMAX:
+            raise NotImplementedError(f'l = {lmax} shells: the GPU path supports l <= {LMAX}')
+        if getattr(self.mol, 'omega', 0) not in (0, 0.):
+            raise NotImplementedError('range-separated Coulomb (omega != 0)')
+        self.h_shls = []
+        nbas = mol.nbas
+        d = {}
+        d['bas'] = torch.as_tensor(mol._bas.astype(np.int32), device=dev)
+        d['env'] = torch.as_tensor(mol.env_kernel, device=dev)
+        d['ao_loc'] = torch.as_tensor(mol.ao_loc.astype(np.int32), device=dev)
+        d['coeff'] = torch.as_tensor(mol.coeff, device=dev)
+        rys_buf, rys = clib.rys_table(dev)
+        d['rys_buf'] = rys_buf
+        self._rys = rys
+        stream = _stream_ptr()
+
+        # shell-pair overlap mask (float32 log bound > log tol^2), lower triangle
+        exps, cs = mol.diffuse_pgto()
+        exps_d = torch.as_tensor(exps.astype(np.float32), device=dev)
+        logc_d = torch.as_tensor(np.log(np.abs(cs.astype(np.float32))), device=dev)
+        coords_d = torch.as_tensor(mol.bas_coords(), device=dev)
+        ls_d = torch.as_tensor(mol._bas[:, 1].astype(np.int32), device=dev)
+        mask_d = torch.empty((nbas, nbas), dtype=torch.int8, device=dev)
+        clib.check(lib.jk_ovlp_mask(_ptr(mask_d), _ptr(exps_d), _ptr(logc_d), _ptr(coords_d),
+                                    _ptr(ls_d), ctypes.c_int(nbas),
+                                    ctypes.c_float(math.log(self.direct_scf_tol**2)), stream),
+                   'jk_ovlp_mask')
+        mask = mask_d.cpu().numpy().astype(bool).ravel()
+
+        # tile-ordered pair lists of each group pair, masked
+        offs = mol.group_offsets
+        ng = len(offs) - 1
+        keys = [(i, j) for i in range(ng) for j in range(i + 1)]
+        plist = []
+        for i, j in keys:
+            p = tile_pairs(offs[i], offs[i+1], offs[j], offs[j+1], nbas, self.tile)
+            plist.append(p[mask[p]])
+        allp = np.concatenate(plist).astype(np.uint32) if plist else np.zeros(0, np.uint32)
+        q_all = np.zeros(len(allp), dtype=np.float32)
+        if len(allp):
+            allp_d = torch.as_tensor(allp.astype(np.int64), device=dev).to(torch.int32)
+            q_d = torch.empty(len(allp), dtype=torch.float32, device=dev)
+            clib.check(lib.jk_pair_qcond(_ptr(q_d), _ptr(allp_d), ctypes.c_int(len(allp)),
+                                         _ptr(d['bas']), _ptr(d['env']), ctypes.c_int(nbas),
+                                         rys, stream), 'jk_pair_qcond')
+            q_all = q_d.cpu().numpy()
+        split = np.arange(math.log(self.direct_scf_tol), 2., Q_COND_MARGIN)
+        cache = {}
+        dev_pairs, dev_q = {}, {}
+        pos = 0
+        for (i, j), p in zip(keys, plist):
+            q = q_all[pos:pos+len(p)]
+            pos += len(p)
+            idx = group_by_split_points(q, split)
+            pij = p[idx].astype(np.uint32)
+            qij = q[idx].astype(np.float32)
+            cache[i, j] = (pij, qij, qij)
+            dev_pairs[i, j] = torch.as_tensor(pij.astype(np.int64), device=dev).to(torch.int32)
+            dev_q[i, j] = torch.as_tensor(qij, device=dev)
+        self.bas_pair_cache = cache
+        d['pairs'] = dev_pairs
+        d['q'] = dev_q
+        self._dev = d
+        self.device = dev
+        self.timings['build'] = time.perf_counter() - t0
+        return self
+
+    # -- AO transforms (gpu4pyscf/scf/jk.py:444-460) ---------------------
+    @property
+    def coeff(self):
+        return self.sorted_mol.coeff
+
+    def apply_coeff_C_mat_CT(self, mat):
+        '''original AO -> sorted Cartesian AO: C M C^T over the last two axes'''
+        if self._dev is None:
+            self.build()
+        C = self._dev['coeff']
+        return C @ mat @ C.T
+
+    def apply_coeff_CT_mat_C(self, mat):
+        '''sorted Cartesian AO -> original AO: C^T M C over the last two axes'''
+        if self._dev is None:
+            self.build()
+        C = self._dev['coeff']
+        return C.T @ mat @ C
+
+    # -- J/K on the sorted basis (gpu4pyscf/scf/jk.py:492-641) -----------
+    def tasks(self):
+        '''Class tasks (i, j, k, l) over angular groups, reference order
+        (gpu4pyscf/scf/jk.py:520-524); empty pair lists are skipped.'''
+        mol = self.sorted_mol
+        ng = len(mol.l_ctr_counts)
+        out = []
+        for i in range(ng):
+            for j in range(i + 1):
+                for k in range(i + 1):
+                    for l in range(k + 1):
+                        if len(self.bas_pair_cache[i, j][0]) and len(self.bas_pair_cache[k, l][0]):
+                            out.append((i, j, k, l))
+        return out
+
+    def get_jk(self, dms, hermi, verbose=None, with_j=True, with_k=True,
+               count_quartets=False):
+        '''J/K for density matrices ``dms`` (n_dm, nao_c, nao_c) in the sorted
+        Cartesian basis (torch, float64, on the device).'''
+        torch = _torch()
+        if self._dev is None:
+            self.build()
+        if callable(dms):
+            dms = dms()
+        lib = clib.load()
+        mol = self.sorted_mol
+        d = self._dev
+        nbas, nao = mol.nbas, mol.nao
+        assert dms.ndim == 3 and dms.shape[-1] == nao
+        dms = dms.contiguous()
+        stream = _stream_ptr()
+        n_dm0 = dms.shape[0]
+        dm_cond = torch.empty((nbas, nbas), dtype=torch.float32, device=dms.device)
+        work = torch.empty((nbas, nbas), dtype=torch.float64, device=dms.device)
+        clib.check(lib.jk_dm_cond(_ptr(dm_cond), _ptr(work), _ptr(dms), ctypes.c_int(n_dm0),
+                                  ctypes.c_int(nao), _ptr(d['ao_loc']), ctypes.c_int(nbas),
+                                  ctypes.c_int(hermi), stream), 'jk_dm_cond')
+        dm_penalty = float(dm_cond.max())
+        if hermi == 0:
+            dms = torch.cat([dms, dms.transpose(1, 2)]).contiguous()
+        n_dm = dms.shape[0]
+        vj = torch.zeros_like(dms) if with_j else None
+        vk = torch.zeros_like(dms) if with_k else None
+        counter = torch.zeros(1, dtype=torch.int64, device=dms.device) if count_quartets else None
+
+        args = clib.JKBuildArgs()
+        args.abi_version = clib.JK_ABI_VERSION
+        args.nbas = nbas
+        args.nao = nao
+        args.bas = d['bas'].data_ptr()
+        args.env = d['env'].data_ptr()
+        args.ao_loc = d['ao_loc'].data_ptr()
+        args.dm_cond = dm_cond.data_ptr()
+        args.cutoff = np.float32(math.log(self.direct_scf_tol))
+        args.dm_penalty = dm_penalty
+        args.n_dm = n_dm
+        args.dm = dms.data_ptr()
+        args.vj = vj.data_ptr() if with_j else None
+        args.vk = vk.data_ptr() if with_k else None
+        args.rys = self._rys
+        args.n_quartets = counter.data_ptr() if counter is not None else None
+        uniq = mol.uniq_l_ctr
+        for i, j, k, l in self.tasks():
+            pij, qij = d['pairs'][i, j], d['q'][i, j]
+            pkl, qkl = d['pairs'][k, l], d['q'][k, l]
+            args.li, args.lj, args.lk, args.ll = (int(uniq[x, 0]) for x in (i, j, k, l))
+            args.iprim, args.jprim, args.kprim, args.lprim = (int(uniq[x, 1]) for x in (i, j, k, l))
+            args.npairs_ij = len(pij)
+            args.npairs_kl = len(pkl)
+            args.pairs_ij = pij.data_ptr()
+            args.pairs_kl = pkl.data_ptr()
+            args.q_ij = qij.data_ptr()
+            args.q_kl = qkl.data_ptr()
+            st = lib.jk_build(ctypes.byref(args), stream)
+            if st != 0:
+                clib.check(st, f'jk_build ({"spdfg"[args.li]}{"spdfg"[args.lj]}|'
+                               f'{"spdfg"[args.lk]}{"spdfg"[args.ll]})')
+        if counter is not None:
+            self.last_quartets = int(counter.item())
+        if hermi == 1:
+            if with_j:
+                vj *= 2.
+            if with_k:
+                vk = vk + vk.transpose(1, 2)
+        elif hermi == 2:
+            if with_j:
+                vj.zero_()
+            if with_k:
+                vk = vk - vk.transpose(1, 2)
+        else:
+            h = n_dm // 2
+            if with_j:
+                vj = vj[:h] + vj[h:].transpose(1, 2)
+            if with_k:
+                vk = vk[:h] + vk[h:].transpose(1, 2)
+        if with_j:
+            vj = vj + vj.transpose(1, 2)
+        return vj, vk
+
+
+def _to_device(dm):
+    torch = _torch()
+    if isinstance(dm, torch.Tensor):
+        return dm.to(device=_device(), dtype=torch.float64), True
+    return torch.as_tensor(np.asarray(dm, dtype=np.float64), device=_device()), False
+
+
+def get_jk(mol, dm, hermi=0, vhfopt=None, with_j=True, with_k=True,
+           omega=None, lr_factor=None, sr_factor=None, verbose=None):
+    '''Coulomb and exchange matrices, gpu4pyscf/scf/jk.py:60-151.
+
+    Returns (vj, vk) with the shape of ``dm``; an element is None when not
+    requested.'''
+    assert with_j or with_k
+    if omega is None:
+        omega = getattr(mol, 'omega', 0.)
+    if omega != 0 or lr_factor is not None or sr_factor is not None:
+        raise NotImplementedError('range-separated Coulomb (omega != 0)')
+    if vhfopt is None:
+        vhfopt = _VHFOpt(mol).build()
+    dm_t, was_torch = _to_device(dm)
+    shape = dm_t.shape
+    nao0 = shape[-1]
+    dms = dm_t.reshape(-1, nao0, nao0)
+    dms = vhfopt.apply_coeff_C_mat_CT(dms)
+    vj, vk = vhfopt.get_jk(dms, hermi, verbose, with_j=with_j, with_k=with_k)
+    if with_j:
+        vj = vhfopt.apply_coeff_CT_mat_C(vj).reshape(shape)
+    if with_k:
+        vk = vhfopt.apply_coeff_CT_mat_C(vk).reshape(shape)
+    if not was_torch:
+        vj = vj.cpu().numpy() if with_j else None
+        vk = vk.cpu().numpy() if with_k else None
+    return vj, vk
+
+
+def get_j(mol, dm, hermi=0, vhfopt=None, verbose=None):
+    '''Coulomb matrix, gpu4pyscf/scf/jk.py:206-230.'''
+    return get_jk(mol, dm, hermi, vhfopt, with_j=True, with_k=False)[0]
+
+
+def get_k(mol, dm, hermi=0, vhfopt=None, omega=None, lr_factor=None, sr_factor=None):
+    '''Exchange matrix, gpu4pyscf/scf/jk.py:153-204.'''
+    return get_jk(mol, dm, hermi, vhfopt, with_j=False, with_k=True, omega=omega,
+                  lr_factor=lr_factor, sr_factor=sr_factor)[1]

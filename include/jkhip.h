@@ -1,0 +1,110 @@
+/*
+ * C-ABI of the MI355X direct-SCF J/K engine (libjkhip.so).
+ *
+ * Plain C: device buffers are passed as raw pointers (e.g. torch
+ * tensor.data_ptr()), HIP streams as void*.  Every entry point returns 0 on
+ * success and a non-zero status otherwise; jk_last_error() gives the message.
+ * The callee never allocates device memory: the caller owns and zeroes
+ * vj/vk, exactly as gpu4pyscf/scf/jk.py:538-539 does for the reference.
+ *
+ * Layout conventions (identical to gpu4pyscf/lib/gvhf-rys/vhf.cuh:5-24):
+ *   bas: nbas x 8 int32 (ATOM_OF, ANG_OF, NPRIM_OF, NCTR_OF, KAPPA_OF,
+ *        PTR_EXP, PTR_COEFF, PTR_BAS_COORD); env: float64 with s/p
+ *        coefficients already scaled by sqrt((2l+1)/4pi)
+ *        (gpu4pyscf/gto/mole.py:1120-1132); ao_loc: nbas+1 int32.
+ *   dm, vj, vk: n_dm x nao x nao float64, row major, sorted Cartesian AOs.
+ *   pair ids: ish*nbas + jsh (uint32) with ish >= jsh.
+ */
+#ifndef JKHIP_H
+#define JKHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JK_ABI_VERSION  1
+#define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
+#define JK_LMAX         3       /* max angular momentum on the GPU path */
+
+/* Chebyshev tables of Rys roots/weights (device pointer + layout).
+ * coef[off[n-1] + (k*2n + f)*(degree+1) + d] on unit intervals [k, k+1),
+ * k < xmax[n-1]; above, u_i = large[n*(n-1)/2 + i] / x and
+ * w_i = large[large_w + n*(n-1)/2 + i] / sqrt(x). */
+typedef struct {
+    const double *coef;
+    int64_t off[JK_RYS_NMAX];
+    double xmax[JK_RYS_NMAX];
+    int64_t large_r;            /* offset of large-x nodes in coef */
+    int64_t large_w;            /* offset of large-x weights in coef */
+    int degree;
+} JKRysTable;
+
+/* One angular-class task of the J/K build.
+ * Replaces RYS_build_jk (gpu4pyscf/lib/gvhf-rys/rys_contract_jk.cu:344-350),
+ * which the reference binds at gpu4pyscf/scf/jk.py:43 and calls per task at
+ * gpu4pyscf/scf/jk.py:554-585. */
+typedef struct {
+    int abi_version;
+    int nbas, nao;
+    const int32_t *bas;             /* device */
+    const double *env;              /* device */
+    const int32_t *ao_loc;          /* device */
+    /* class: all shells of a group share l and nprim */
+    int li, lj, lk, ll;
+    int iprim, jprim, kprim, lprim;
+    /* screened pair lists of the two group pairs, in descending-q bins */
+    int npairs_ij, npairs_kl;
+    const uint32_t *pairs_ij, *pairs_kl;    /* device */
+    const float *q_ij, *q_kl;               /* device, log sqrt max|(ij|ij)| */
+    const float *dm_cond;                   /* device, nbas x nbas */
+    float cutoff;                           /* log(direct_scf_tol) */
+    float dm_penalty;                       /* max(dm_cond) */
+    int n_dm;
+    const double *dm;                       /* device */
+    double *vj;                             /* device or NULL */
+    double *vk;                             /* device or NULL */
+    JKRysTable rys;
+    unsigned long long *n_quartets;         /* device counter or NULL */
+} JKBuildArgs;
+
+/* Accumulate one class task into vj/vk (before the hermi post-processing of
+ * gpu4pyscf/scf/jk.py:594-616).  stream: hipStream_t or NULL. */
+int jk_build(const JKBuildArgs *args, void *stream);
+
+/* float32 shell-pair overlap mask (lower triangle), replaces
+ * PBCovlp_mask_estimation (gpu4pyscf/lib/pbc/overlap.cu:2378-2387) as used by
+ * _shell_overlap_mask (gpu4pyscf/pbc/gto/int1e.py:403-428).
+ * exps, log_coeff: per-shell most diffuse primitive (float32, device);
+ * coords: nbas x 3 float64 (device); ls: nbas int32 (device);
+ * mask: nbas x nbas int8 (device, written fully). */
+int jk_ovlp_mask(int8_t *mask, const float *exps, const float *log_coeff,
+                 const double *coords, const int32_t *ls, int nbas,
+                 float log_cutoff, void *stream);
+
+/* q[p] = float32(log(max |(ij|ij)|) / 2), -700 if zero, for pairs p.
+ * Replaces int2e_qcond_estimator (gpu4pyscf/lib/gvhf-rys/
+ * nr_sr_estimator.cu:416-430) for omega = 0; evaluated in FP64. */
+int jk_pair_qcond(float *q, const uint32_t *pairs, int npairs,
+                  const int32_t *bas, const double *env, int nbas,
+                  JKRysTable rys, void *stream);
+
+/* dm_cond[ish,jsh] = float32(log(max_{dm, u in ish, v in jsh} |D| + 1e-300)),
+ * with the block maxima symmetrised (A + A^T) first when hermi == 0.
+ * Replaces condense('absmax') + log (gpu4pyscf/scf/jk.py:510-515,
+ * gpu4pyscf/lib/cupy_helper.py:1068-1184).  work: nbas*nbas float64. */
+int jk_dm_cond(float *dm_cond, double *work, const double *dm, int n_dm,
+               int nao, const int32_t *ao_loc, int nbas, int hermi,
+               void *stream);
+
+/* Last error message of this thread ("" if none). */
+const char *jk_last_error(void);
+
+/* ABI version compiled into the library (JK_ABI_VERSION). */
+int jk_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
