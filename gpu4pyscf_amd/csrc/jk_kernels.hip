@@ -364,6 +364,11 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk)
             __syncthreads();
         }
         bool flush = (base + WAVE >= wi.kl1) || stop;
+        if (qn == 0) {
+            // nothing queued: the queue holds no valid entry to read
+            if (flush) break;
+            continue;
+        }
         if (qn < WAVE && !flush) continue;
         int nb = min(qn, WAVE);
         // ---- evaluate up to 64 queued quartets, one per lane
