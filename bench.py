@@ -1,0 +1,159 @@
+'''Benchmark of the direct-SCF J/K engine on one MI355X.
+
+Workload (default): 32-water cluster (4x4x2 lattice, 3.0 Angstrom spacing,
+seed 0), def2-TZVP (1376 spherical AOs); one step = one incremental
+get_jk(dD, hermi=1) with dD = 1e-3 (G + G^T)/2, G ~ N(0,1) seeded, through
+the full drop-in boundary (AO transforms, dm_cond, all class launches,
+hermi post-processing).  Inputs are resident in HBM before timing.
+
+Prints one JSON line: value = surviving shell quartets per second.
+'''
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def water_mol(shape, spacing, basis, seed=0):
+    from gpu4pyscf_amd.gto import geometry as G
+    from gpu4pyscf_amd.gto.mole import M
+    return M(atom=G.to_string(G.water_cluster(shape, spacing, seed=seed)), basis=basis)
+
+
+def cpu_baseline(opt, dms_c, target_s=15.):
+    '''Oracle (OpenMP C restatement) quartets/s on an ij-strided sample of
+    the same J/K build.'''
+    from oracle import oracle as O
+    sm = opt.sorted_mol
+    cache = {k: (v[0], v[1]) for k, v in opt.bas_pair_cache.items()}
+    dms = dms_c.cpu().numpy()
+    stride = 4096
+    t = 0.
+    while True:
+        t0 = time.perf_counter()
+        _, _, n, _ = O.jk_sorted(sm, dms, 1, cache, opt.direct_scf_tol, ij_stride=stride)
+        t = time.perf_counter() - t0
+        if t > target_s / 4 or stride == 1:
+            break
+        stride = max(1, int(stride * max(t, 1e-3) / target_s))
+    cores = len(os.sched_getaffinity(0))
+    return dict(value=n / t, unit='quartets/s', cores=int(os.environ.get('OMP_NUM_THREADS', cores)),
+                kind='port',
+                sample=f'every {stride}-th ij shell pair of each class task of the same '
+                       f'get_jk ({n} quartets, {t:.1f} s)')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--config', default='water32-tzvp')
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported',
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from gpu4pyscf_amd.scf import jk
+    from gpu4pyscf_amd.scf.flops import quartet_flops
+    if args.config == 'water32-tzvp':
+        shape, spacing, basis = (4, 4, 2), 3.0, 'def2-tzvp'
+    elif args.config == 'water8-svp':
+        shape, spacing, basis = (2, 2, 2), 3.0, 'def2-svp'
+    elif args.config == 'water64-tzvp':
+        shape, spacing, basis = (4, 4, 4), 3.1, 'def2-tzvp'
+    else:
+        raise SystemExit(f'unknown config {args.config}')
+    mol = water_mol(shape, spacing, basis)
+    log(f'[bench] {args.config}: nao={mol.nao} nbas={mol.nbas}')
+    dev = torch.device('cuda', 0)
+    t0 = time.perf_counter()
+    opt = jk._VHFOpt(mol).build()
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    log(f'[bench] screening setup {t_build:.2f} s, tasks={len(opt.tasks())}')
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal((mol.nao, mol.nao))
+    ddm = torch.as_tensor(1e-3 * (g + g.T) / 2, device=dev)
+
+    def step():
+        return jk.get_jk(mol, ddm, hermi=1, vhfopt=opt)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log('[bench] warmup done')
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    log(f'[bench] get_jk {dt*1e3:.1f} ms/step')
+
+    # per-class counts and kernel times (separate, untimed pass)
+    dms_c = opt.apply_coeff_C_mat_CT(ddm[None])
+    opt.get_jk(dms_c, 1, count_quartets=True, time_classes=True)
+    counts = opt.last_task_counts
+    nq = opt.last_quartets
+    uniq = opt.sorted_mol.uniq_l_ctr
+    flops = {}
+    for t, c in counts.items():
+        ls = tuple(int(uniq[x, 0]) for x in t)
+        ps = tuple(int(uniq[x, 1]) for x in t)
+        flops[t] = c * quartet_flops(ls, ps, 1)
+    ms = dict(opt.last_task_ms)
+    kern_ms = sum(ms.values())
+    total_flops = sum(flops.values())
+    # dominant angular class by kernel time
+    by_class = {}
+    for t, v in ms.items():
+        key = tuple(int(uniq[x, 0]) for x in t)
+        a = by_class.setdefault(key, [0., 0., 0])
+        a[0] += v; a[1] += flops[t]; a[2] += 1
+    dom = max(by_class, key=lambda k: by_class[k][0])
+    dom_ms, dom_flops, dom_n = by_class[dom]
+    peak = 78.6
+    log(f'[bench] quartets={nq} kernels {kern_ms:.1f} ms, {total_flops/kern_ms/1e9:.2f} TFLOP/s '
+        f'model; dominant class {dom} {dom_ms:.1f} ms')
+    for key in sorted(by_class, key=lambda k: -by_class[k][0])[:12]:
+        v = by_class[key]
+        log(f'    class {key}: {v[0]:.2f} ms  {v[1]/max(v[0],1e-9)/1e9:.2f} TFLOP/s  launches {v[2]}')
+    # compute bound: FP64 FMA, whose vector and matrix peaks coincide (78.6 TFLOP/s)
+    roofline = dict(bound='mfma',
+                    kernel='jk class kernels, dominant class (%s)' % ''.join('spdf'[x] for x in dom),
+                    achieved=round(dom_flops / dom_ms / 1e9, 3),
+                    peak=peak, unit='TFLOP/s',
+                    frac=round(dom_flops / dom_ms / 1e9 / peak, 4), traffic=None,
+                    all_classes=dict(achieved=round(total_flops / kern_ms / 1e9, 3),
+                                     frac=round(total_flops / kern_ms / 1e9 / peak, 4),
+                                     kernel_ms=round(kern_ms, 3)))
+    out = dict(metric='J/K surviving shell quartets per second (one incremental get_jk, FP64)',
+               value=nq / dt, unit='quartets/s', n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=dt * 1e3, higher_is_better=True,
+               scaling='weak', vs_baseline=None, dtype='f64', data='synthetic',
+               config=dict(workload=f'{args.config}: one get_jk(dD, hermi=1), '
+                                    f'nao={mol.nao}, quartets={nq}',
+                           global_batch=1, seq_len=0, parallelism='single GPU'),
+               roofline=roofline, setup_s=t_build)
+    if not args.no_cpu:
+        log('[bench] CPU baseline (oracle) ...')
+        out['cpu_baseline'] = cpu_baseline(opt, dms_c)
+        log(f"[bench] CPU baseline {out['cpu_baseline']['value']:.3e} quartets/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -205,9 +205,14 @@ class _VHFOpt:
         return out
 
     def get_jk(self, dms, hermi, verbose=None, with_j=True, with_k=True,
-               count_quartets=False):
+               count_quartets=False, time_classes=False):
         '''J/K for density matrices ``dms`` (n_dm, nao_c, nao_c) in the sorted
-        Cartesian basis (torch, float64, on the device).'''
+        Cartesian basis (torch, float64, on the device).
+
+        count_quartets: store the surviving quartets of each class task in
+        ``self.last_task_counts`` (and their sum in ``self.last_quartets``).
+        time_classes: store (task, milliseconds) of each class launch, timed
+        with HIP events on the current stream, in ``self.last_task_ms``.'''
         torch = _torch()
         if self._dev is None:
             self.build()
@@ -232,7 +237,10 @@ class _VHFOpt:
         n_dm = dms.shape[0]
         vj = torch.zeros_like(dms) if with_j else None
         vk = torch.zeros_like(dms) if with_k else None
-        counter = torch.zeros(1, dtype=torch.int64, device=dms.device) if count_quartets else None
+        tasks = self.tasks()
+        counter = (torch.zeros(len(tasks), dtype=torch.int64, device=dms.device)
+                   if count_quartets else None)
+        events = []
 
         args = clib.JKBuildArgs()
         args.abi_version = clib.JK_ABI_VERSION
@@ -249,9 +257,10 @@ class _VHFOpt:
         args.vj = vj.data_ptr() if with_j else None
         args.vk = vk.data_ptr() if with_k else None
         args.rys = self._rys
-        args.n_quartets = counter.data_ptr() if counter is not None else None
         uniq = mol.uniq_l_ctr
-        for i, j, k, l in self.tasks():
+        for t, (i, j, k, l) in enumerate(tasks):
+            if counter is not None:
+                args.n_quartets = counter.data_ptr() + 8 * t
             pij, qij = d['pairs'][i, j], d['q'][i, j]
             pkl, qkl = d['pairs'][k, l], d['q'][k, l]
             args.li, args.lj, args.lk, args.ll = (int(uniq[x, 0]) for x in (i, j, k, l))
@@ -262,12 +271,23 @@ class _VHFOpt:
             args.pairs_kl = pkl.data_ptr()
             args.q_ij = qij.data_ptr()
             args.q_kl = qkl.data_ptr()
+            if time_classes:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
             st = lib.jk_build(ctypes.byref(args), stream)
+            if time_classes:
+                e1.record()
+                events.append(((i, j, k, l), e0, e1))
             if st != 0:
                 clib.check(st, f'jk_build ({"spdfg"[args.li]}{"spdfg"[args.lj]}|'
                                f'{"spdfg"[args.lk]}{"spdfg"[args.ll]})')
         if counter is not None:
-            self.last_quartets = int(counter.item())
+            self.last_task_counts = dict(zip(tasks, counter.cpu().numpy().tolist()))
+            self.last_quartets = int(sum(self.last_task_counts.values()))
+        if time_classes:
+            torch.cuda.synchronize()
+            self.last_task_ms = [(t, e0.elapsed_time(e1)) for t, e0, e1 in events]
         if hermi == 1:
             if with_j:
                 vj *= 2.

@@ -173,12 +173,9 @@ def build_pairs(sorted_mol, direct_scf_tol=1e-13, tile=12, q_cond_fn=None):
 def dm_condition(dms, ao_loc, hermi):
     '''float32 log(max |D| per shell block + 1e-300) and its maximum
     (gpu4pyscf/scf/jk.py:510-516).'''
-    nbas = len(ao_loc) - 1
     a = np.abs(dms).max(axis=0)
-    cond = np.zeros((nbas, nbas))
-    for i in range(nbas):
-        for j in range(nbas):
-            cond[i, j] = a[ao_loc[i]:ao_loc[i+1], ao_loc[j]:ao_loc[j+1]].max()
+    starts = np.asarray(ao_loc[:-1])
+    cond = np.maximum.reduceat(np.maximum.reduceat(a, starts, axis=0), starts, axis=1)
     if hermi == 0:
         cond = cond + cond.T
     cond = np.log(cond + 1e-300).astype(np.float32)
@@ -196,7 +193,7 @@ def flatten_pairs(cache, ngroups):
 
 
 def jk_sorted(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13,
-              with_j=True, with_k=True, list_quartets=0):
+              with_j=True, with_k=True, list_quartets=0, ij_stride=1):
     '''J/K on the sorted Cartesian basis including the hermi post-processing.
     Returns (vj, vk, n_surviving, quartet_list or None).'''
     dms = np.asarray(dms, dtype=np.float64)
@@ -228,7 +225,7 @@ def jk_sorted(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13,
         ctypes.c_int(ng), _p(off), _p(pairs), _p(q), _p(cond),
         ctypes.c_float(cutoff), ctypes.c_float(penalty),
         ctypes.c_int(int(with_j)), ctypes.c_int(int(with_k)),
-        qptr, ctypes.c_long(list_quartets))
+        qptr, ctypes.c_long(list_quartets), ctypes.c_int(ij_stride))
     if hermi == 1:
         vj *= 2.
         vk = vk + vk.transpose(0, 2, 1)
@@ -261,7 +258,8 @@ def survivors(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13):
                         _p(bas), ctypes.c_int(sorted_mol.nbas), _p(env), _p(aol),
                         ctypes.c_int(ng), _p(off), _p(pairs), _p(q), _p(cond),
                         ctypes.c_float(cutoff), ctypes.c_float(penalty),
-                        ctypes.c_int(0), ctypes.c_int(0), _p(out), ctypes.c_long(cap))
+                        ctypes.c_int(0), ctypes.c_int(0), _p(out), ctypes.c_long(cap),
+                        ctypes.c_int(1))
         if n <= cap:
             out = out[:n]
             break

@@ -343,13 +343,15 @@ static inline int survives(float q_ij, float q_kl, float cutoff, float dm_penalt
  * pairs[pair_off[gi*(gi+1)/2+gj] : pair_off[..+1]] with q_cond alongside.
  * Returns the number of surviving quartets; if quartets != NULL the first
  * max_quartets survivors (bas_ij, bas_kl) are written there in task order.
+ * ij_stride > 1 evaluates only every ij_stride-th ij pair of each task (a
+ * bounded sample used to time the CPU baseline).
  */
 long oracle_jk(int n_dm, int nao, const double *dms, double *vj, double *vk,
                const int *bas, int nbas, const double *env, const int *ao_loc,
                int ngroups, const int *pair_off, const uint32_t *pairs,
                const float *q_cond, const float *dm_cond, float cutoff,
                float dm_penalty, int with_j, int with_k,
-               uint32_t *quartets, long max_quartets)
+               uint32_t *quartets, long max_quartets, int ij_stride)
 {
     size_t nao2 = (size_t)nao * nao;
     long total = 0;
@@ -402,7 +404,7 @@ long oracle_jk(int n_dm, int nao, const double *dms, double *vj, double *vk,
             int pij0 = pair_off[gi*(gi+1)/2+gj], pij1 = pair_off[gi*(gi+1)/2+gj+1];
             int pkl0 = pair_off[gk*(gk+1)/2+gl], pkl1 = pair_off[gk*(gk+1)/2+gl+1];
 #pragma omp for schedule(dynamic, 1) nowait
-            for (int pij = pij0; pij < pij1; pij++) {
+            for (int pij = pij0; pij < pij1; pij += ij_stride) {
                 uint32_t bij = pairs[pij];
                 int ish = bij / nbas, jsh = bij % nbas;
                 for (int pkl = pkl0; pkl < pkl1; pkl++) {
