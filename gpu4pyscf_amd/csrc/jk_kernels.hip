@@ -63,6 +63,19 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Inclusive segmented sum over lanes with equal, contiguous segment ids:
+// the last lane of a segment receives the segment total.
+__device__ __forceinline__ double seg_sum(double v, int seg, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        double u = __shfl_up(v, off, WAVE);
+        int s = __shfl_up(seg, off, WAVE);
+        if (lane >= off && s == seg) v += u;
+    }
+    return v;
+}
+
 // Rys nodes u_i = t_i^2 and weights on the Chebyshev tables.
 template <int N>
 __device__ __forceinline__ void rys_roots(const JKRysTable &tab, double x,
@@ -237,12 +250,17 @@ __device__ __forceinline__ WorkItem work_item(int npairs_kl, int kl_chunk)
 }
 
 // ---------------------------------------------------------------- per-lane
+// Primitive-pair record, see jk_pair_data(): (a, 1/a, Px, Py, Pz, K).
+#define PREC 6
+
 // One quartet per lane: returns the Cartesian block (times the symmetry and
 // normalisation prefactors) in gout[((i*nfj+j)*nfk+k)*nfl+l].
+// pij: records of the (uniform) ij pair; pkl: records of this lane's kl pair.
 template <int LI, int LJ, int LK, int LL>
-__device__ __forceinline__ void eri_lane(double *gout, const ShellPair &sp, int iprim, int jprim,
-                                         int kprim, int lprim, int ksh, int lsh,
-                                         const int32_t *bas, const double *env,
+__device__ __forceinline__ void eri_lane(double *gout, const double *pij, int nij,
+                                         const double *pkl, int nkl,
+                                         const double *ri, const double *rij,
+                                         const double *rk, const double *rkl,
                                          const JKRysTable &rys, double fac_sym)
 {
     constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
@@ -255,57 +273,40 @@ __device__ __forceinline__ void eri_lane(double *gout, const ShellPair &sp, int 
     constexpr CartPow<LL> cl{};
 #pragma unroll
     for (int n = 0; n < NF; ++n) gout[n] = 0.;
-    const int32_t *bk = bas + ksh * 8, *bl = bas + lsh * 8;
-    const double *rk = env + bk[7], *rl = env + bl[7];
-    double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
-    double rr_ij = sp.rij[0]*sp.rij[0] + sp.rij[1]*sp.rij[1] + sp.rij[2]*sp.rij[2];
-    double rr_kl = rkl[0]*rkl[0] + rkl[1]*rkl[1] + rkl[2]*rkl[2];
-    const double *ek = env + bk[5], *el = env + bl[5];
-    const double *cck = env + bk[6], *ccl = env + bl[6];
-    for (int kp = 0; kp < kprim; ++kp)
-    for (int lp = 0; lp < lprim; ++lp) {
-        double ak = ek[kp], al = el[lp];
-        double akl = ak + al;
-        double al_akl = al / akl;
-        double ckcl = cck[kp] * ccl[lp] * exp(-ak * al_akl * rr_kl);
-        double Q[3];
+    for (int kp = 0; kp < nkl; ++kp) {
+        const double *q = pkl + kp * PREC;
+        const double akl = q[0], rakl = q[1];
+        const double Q[3] = {q[2], q[3], q[4]};
+        const double Kkl = q[5] * fac_sym;
+        double QC[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) Q[d] = rk[d] + rkl[d] * al_akl;
-        for (int ip = 0; ip < iprim; ++ip)
-        for (int jp = 0; jp < jprim; ++jp) {
-            double ai = env[sp.pexpi + ip], aj = env[sp.pexpj + jp];
-            double aij = ai + aj;
-            double aj_aij = aj / aij;
-            double cicj = env[sp.pci + ip] * env[sp.pcj + jp] * exp(-ai * aj_aij * rr_ij);
-            double PQ[3], PA[3];
+        for (int d = 0; d < 3; ++d) QC[d] = Q[d] - rk[d];
+        for (int ip = 0; ip < nij; ++ip) {
+            const double *p = pij + ip * PREC;
+            const double aij = p[0], raij = p[1];
+            double PA[3], PQ[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                PA[d] = sp.rij[d] * aj_aij;
-                PQ[d] = sp.ri[d] + PA[d] - Q[d];
-            }
+            for (int d = 0; d < 3; ++d) { PA[d] = p[2+d] - ri[d]; PQ[d] = p[2+d] - Q[d]; }
             double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
-            double aijkl = aij + akl;
-            double theta = aij * akl / aijkl;
-            double fac = PI_FAC * fac_sym * cicj * ckcl / (aij * akl * sqrt(aijkl));
+            double r_aa = 1. / (aij + akl);
+            double theta = aij * akl * r_aa;
+            double fac = p[5] * Kkl * sqrt(r_aa);
             double rt[N], wt[N];
             rys_roots<N>(rys, theta * rr, rt, wt);
 #pragma unroll
             for (int ir = 0; ir < N; ++ir) {
-                double rt_aa = rt[ir] / aijkl;
+                double rt_aa = rt[ir] * r_aa;
                 double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
-                double b10 = .5 / aij * (1. - rt_aij);
-                double b01 = .5 / akl * (1. - rt_akl);
+                double b10 = .5 * raij * (1. - rt_aij);
+                double b01 = .5 * rakl * (1. - rt_akl);
                 double b00 = .5 * rt_aa;
                 double gx[GS], gy[GS], gz[GS];
-                build_g<LI, LJ, LK, LL>(gx, 1., PA[0] - rt_aij * PQ[0],
-                                        rkl[0] * al_akl + rt_akl * PQ[0],
-                                        b10, b01, b00, sp.rij[0], rkl[0]);
-                build_g<LI, LJ, LK, LL>(gy, 1., PA[1] - rt_aij * PQ[1],
-                                        rkl[1] * al_akl + rt_akl * PQ[1],
-                                        b10, b01, b00, sp.rij[1], rkl[1]);
+                build_g<LI, LJ, LK, LL>(gx, 1., PA[0] - rt_aij * PQ[0], QC[0] + rt_akl * PQ[0],
+                                        b10, b01, b00, rij[0], rkl[0]);
+                build_g<LI, LJ, LK, LL>(gy, 1., PA[1] - rt_aij * PQ[1], QC[1] + rt_akl * PQ[1],
+                                        b10, b01, b00, rij[1], rkl[1]);
                 build_g<LI, LJ, LK, LL>(gz, wt[ir] * fac, PA[2] - rt_aij * PQ[2],
-                                        rkl[2] * al_akl + rt_akl * PQ[2],
-                                        b10, b01, b00, sp.rij[2], rkl[2]);
+                                        QC[2] + rt_akl * PQ[2], b10, b01, b00, rij[2], rkl[2]);
 #pragma unroll
                 for (int i = 0; i < NFI; ++i)
 #pragma unroll
@@ -328,7 +329,7 @@ __device__ __forceinline__ void eri_lane(double *gout, const ShellPair &sp, int 
 
 template <int LI, int LJ, int LK, int LL>
 __global__ __launch_bounds__(WAVE)
-void jk_lane_kernel(JKBuildArgs a, int kl_chunk)
+void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
 {
     constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
     constexpr int NF = NFI * NFJ * NFK * NFL;
@@ -342,6 +343,14 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk)
     ShellPair sp;
     load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
     const size_t nao2 = (size_t)nao * nao;
+    // K_il / K_jl rows of this work item accumulated in LDS (one density)
+    extern __shared__ double kacc_dyn[];
+    double *kacc = use_lds ? kacc_dyn : nullptr;
+    const int nacc = (NFI + NFJ) * a.l_nao;
+    if (kacc) {
+        for (int n = lane; n < nacc; n += WAVE) kacc[n] = 0.;
+        __syncthreads();
+    }
     int qn = 0;
     unsigned long long nsurv = 0;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -357,7 +366,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk)
                                 sp.ish, sp.jsh, nbas, a.dm_cond);
             }
             uint64_t m = __ballot(keep);
-            if (keep) queue[qn + __popcll(m & lt_mask)] = a.pairs_kl[pkl];
+            if (keep) queue[qn + __popcll(m & lt_mask)] = pkl;
             qn += __popcll(m);
             nsurv += __popcll(m);
             stop = __ballot(fail_margin) != 0;
@@ -374,23 +383,38 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk)
         // ---- evaluate up to 64 queued quartets, one per lane
         double vij[NFI*NFJ];
         bool active = lane < nb;
-        uint32_t bas_kl = queue[active ? lane : 0];
+        const int pkl = queue[active ? lane : 0];
+        uint32_t bas_kl = a.pairs_kl[pkl];
         int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
         double fac_sym = 1.;
         if (sp.ish == sp.jsh) fac_sym *= .5;
         if (ksh == lsh) fac_sym *= .5;
         if (bas_ij == bas_kl) fac_sym *= .5;
         if (!active) fac_sym = 0.;
+        const double *rk = a.env + a.bas[ksh * 8 + 7], *rl = a.env + a.bas[lsh * 8 + 7];
+        const double rk3[3] = {rk[0], rk[1], rk[2]};
+        const double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
+        const int nij = a.iprim * a.jprim, nkl = a.kprim * a.lprim;
         double gout[NF];
-        eri_lane<LI, LJ, LK, LL>(gout, sp, a.iprim, a.jprim, a.kprim, a.lprim,
-                                 ksh, lsh, a.bas, a.env, a.rys, fac_sym);
+        eri_lane<LI, LJ, LK, LL>(gout, a.pdata_ij + (size_t)wi.pij * nij * PREC, nij,
+                                 a.pdata_kl + (size_t)pkl * nkl * PREC, nkl,
+                                 sp.ri, sp.rij, rk3, rkl, a.rys, fac_sym);
         const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
         const int i0 = sp.i0, j0 = sp.j0;
+        // runs of equal k shell are contiguous in the queue (tile order):
+        // segment = run id, from the run heads
+        const int key = active ? ksh : -1;
+        const int key_prev = __shfl_up(key, 1, WAVE);
+        const bool head = lane == 0 || key_prev != key;
+        const uint64_t heads = __ballot(head);
+        const int run = __popcll(heads & (lt_mask | (1ull << lane)));
+        const int run_next = __shfl_down(run, 1, WAVE);
+        const bool tail = active && (lane == nb - 1 || run_next != run);
         for (int id = 0; id < a.n_dm; ++id) {
             const double *D = a.dm + id * nao2;
-            if (a.vk && active) {
+            if (a.vk) {
                 double *K = a.vk + id * nao2;
-                // K_il += (ij|kl) D_jk ; K_ik += (ij|kl) D_jl
+                // K_ik += (ij|kl) D_jl ; K_il += (ij|kl) D_jk
 #pragma unroll
                 for (int i = 0; i < NFI; ++i) {
                     double kil[NFL], kik[NFK];
@@ -398,22 +422,32 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk)
                     for (int l = 0; l < NFL; ++l) kil[l] = 0.;
 #pragma unroll
                     for (int k = 0; k < NFK; ++k) kik[k] = 0.;
+                    if (active) {
 #pragma unroll
-                    for (int j = 0; j < NFJ; ++j)
+                        for (int j = 0; j < NFJ; ++j)
 #pragma unroll
-                    for (int k = 0; k < NFK; ++k)
+                        for (int k = 0; k < NFK; ++k)
 #pragma unroll
-                    for (int l = 0; l < NFL; ++l) {
-                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
-                        kil[l] = fma(g, D[(j0+j)*nao + k0+k], kil[l]);
-                        kik[k] = fma(g, D[(j0+j)*nao + l0+l], kik[k]);
+                        for (int l = 0; l < NFL; ++l) {
+                            double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
+                            kil[l] = fma(g, D[(j0+j)*nao + k0+k], kil[l]);
+                            kik[k] = fma(g, D[(j0+j)*nao + l0+l], kik[k]);
+                        }
                     }
 #pragma unroll
-                    for (int l = 0; l < NFL; ++l) atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
+                    for (int k = 0; k < NFK; ++k) {
+                        double v = seg_sum(kik[k], run, lane);
+                        if (tail) atomic_add(K + (i0+i)*nao + k0+k, v);
+                    }
+                    if (active) {
 #pragma unroll
-                    for (int k = 0; k < NFK; ++k) atomic_add(K + (i0+i)*nao + k0+k, kik[k]);
+                        for (int l = 0; l < NFL; ++l) {
+                            if (kacc) atomicAdd(kacc + i * a.l_nao + l0 + l - a.l_ao0, kil[l]);
+                            else atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
+                        }
+                    }
                 }
-                // K_jl += (ij|kl) D_ik ; K_jk += (ij|kl) D_il
+                // K_jk += (ij|kl) D_il ; K_jl += (ij|kl) D_ik
 #pragma unroll
                 for (int j = 0; j < NFJ; ++j) {
                     double kjl[NFL], kjk[NFK];
@@ -421,20 +455,30 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk)
                     for (int l = 0; l < NFL; ++l) kjl[l] = 0.;
 #pragma unroll
                     for (int k = 0; k < NFK; ++k) kjk[k] = 0.;
+                    if (active) {
 #pragma unroll
-                    for (int i = 0; i < NFI; ++i)
+                        for (int i = 0; i < NFI; ++i)
 #pragma unroll
-                    for (int k = 0; k < NFK; ++k)
+                        for (int k = 0; k < NFK; ++k)
 #pragma unroll
-                    for (int l = 0; l < NFL; ++l) {
-                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
-                        kjl[l] = fma(g, D[(i0+i)*nao + k0+k], kjl[l]);
-                        kjk[k] = fma(g, D[(i0+i)*nao + l0+l], kjk[k]);
+                        for (int l = 0; l < NFL; ++l) {
+                            double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
+                            kjl[l] = fma(g, D[(i0+i)*nao + k0+k], kjl[l]);
+                            kjk[k] = fma(g, D[(i0+i)*nao + l0+l], kjk[k]);
+                        }
                     }
 #pragma unroll
-                    for (int l = 0; l < NFL; ++l) atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
+                    for (int k = 0; k < NFK; ++k) {
+                        double v = seg_sum(kjk[k], run, lane);
+                        if (tail) atomic_add(K + (j0+j)*nao + k0+k, v);
+                    }
+                    if (active) {
 #pragma unroll
-                    for (int k = 0; k < NFK; ++k) atomic_add(K + (j0+j)*nao + k0+k, kjk[k]);
+                        for (int l = 0; l < NFL; ++l) {
+                            if (kacc) atomicAdd(kacc + (NFI + j) * a.l_nao + l0 + l - a.l_ao0, kjl[l]);
+                            else atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
+                        }
+                    }
                 }
             }
             if (a.vj) {
@@ -486,6 +530,17 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk)
         qn -= nb;
         __syncthreads();
         if (flush && qn == 0) break;
+    }
+    if (kacc) {
+        __syncthreads();
+        for (int n = lane; n < nacc; n += WAVE) {
+            double v = kacc[n];
+            if (v != 0.) {
+                int r = n / a.l_nao, c = n - r * a.l_nao;
+                int row = r < NFI ? sp.i0 + r : sp.j0 + r - NFI;
+                atomic_add(a.vk + (size_t)row * nao + a.l_ao0 + c, v);
+            }
+        }
     }
     if (a.n_quartets && lane == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
 }
@@ -552,8 +607,6 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
     const int row = tl;               // (i, j) row owned by this lane
     const bool has_row = team < NQ && row < NROWS;
     const int irow = has_row ? row / NFJ : 0, jrow = has_row ? row % NFJ : 0;
-    double rr_ij = sp.rij[0]*sp.rij[0] + sp.rij[1]*sp.rij[1] + sp.rij[2]*sp.rij[2];
-
     int qn = 0;
     unsigned long long nsurv = 0;
     bool stop = false;
@@ -580,7 +633,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
             __syncthreads();
             int off = qn;
             for (int w = 0; w < wid; ++w) off += wave_cnt[w];
-            if (keep) queue[off + __popcll(m & lt_mask)] = bkl;
+            if (keep) queue[off + __popcll(m & lt_mask)] = pkl;
             int tot = 0;
             for (int w = 0; w < NW; ++w) tot += wave_cnt[w];
             qn += tot;
@@ -594,47 +647,40 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
             // team `team` evaluates queue[head + team]
             const int qidx = head + team;
             const bool active = team < NQ && qidx < qn;
-            uint32_t bas_kl = queue[active ? qidx : head];
+            const int pkl = queue[active ? qidx : head];
+            uint32_t bas_kl = a.pairs_kl[pkl];
             int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
             double fac_sym = 1.;
             if (sp.ish == sp.jsh) fac_sym *= .5;
             if (ksh == lsh) fac_sym *= .5;
             if (bas_ij == bas_kl) fac_sym *= .5;
-            const int32_t *bk = a.bas + ksh * 8, *bl = a.bas + lsh * 8;
-            const double *rk = a.env + bk[7], *rl = a.env + bl[7];
+            const double *rk = a.env + a.bas[ksh * 8 + 7], *rl = a.env + a.bas[lsh * 8 + 7];
             double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
-            double rr_kl = rkl[0]*rkl[0] + rkl[1]*rkl[1] + rkl[2]*rkl[2];
-            const double *ek = a.env + bk[5], *el = a.env + bl[5];
-            const double *cck = a.env + bk[6], *ccl = a.env + bl[6];
+            const int nij = a.iprim * a.jprim, nkl = a.kprim * a.lprim;
+            const double *pdij = a.pdata_ij + (size_t)wi.pij * nij * PREC;
+            const double *pdkl = a.pdata_kl + (size_t)pkl * nkl * PREC;
             double acc[NFKL];
 #pragma unroll
             for (int n = 0; n < NFKL; ++n) acc[n] = 0.;
-            for (int kp = 0; kp < a.kprim; ++kp)
-            for (int lp = 0; lp < a.lprim; ++lp) {
-                double ak = ek[kp], al = el[lp];
-                double akl = ak + al;
-                double al_akl = al / akl;
-                double ckcl = cck[kp] * ccl[lp] * exp(-ak * al_akl * rr_kl);
-                double Q[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) Q[d] = rk[d] + rkl[d] * al_akl;
-                for (int ip = 0; ip < a.iprim; ++ip)
-                for (int jp = 0; jp < a.jprim; ++jp) {
-                    double ai = a.env[sp.pexpi + ip], aj = a.env[sp.pexpj + jp];
-                    double aij = ai + aj;
-                    double aj_aij = aj / aij;
-                    double cicj = a.env[sp.pci + ip] * a.env[sp.pcj + jp]
-                                * exp(-ai * aj_aij * rr_ij);
-                    double PQ[3], PA[3];
+            for (int kp = 0; kp < nkl; ++kp) {
+                const double *qr = pdkl + kp * PREC;
+                const double akl = qr[0], rakl = qr[1];
+                const double Q[3] = {qr[2], qr[3], qr[4]};
+                const double Kkl = qr[5] * fac_sym;
+                for (int ip = 0; ip < nij; ++ip) {
+                    const double *pr = pdij + ip * PREC;
+                    const double aij = pr[0], raij = pr[1];
+                    double PQ[3], PA[3], QC[3];
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        PA[d] = sp.rij[d] * aj_aij;
-                        PQ[d] = sp.ri[d] + PA[d] - Q[d];
+                        PA[d] = pr[2+d] - sp.ri[d];
+                        PQ[d] = pr[2+d] - Q[d];
+                        QC[d] = Q[d] - rk[d];
                     }
                     double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
-                    double aijkl = aij + akl;
-                    double theta = aij * akl / aijkl;
-                    double fac = PI_FAC * fac_sym * cicj * ckcl / (aij * akl * sqrt(aijkl));
+                    double r_aa = 1. / (aij + akl);
+                    double theta = aij * akl * r_aa;
+                    double fac = pr[5] * Kkl * sqrt(r_aa);
                     __syncthreads();
                     if (active && tl == 0) {
                         double rt[N], wt[N];
@@ -645,19 +691,20 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
                     __syncthreads();
                     if (active && tl < 3 * N) {
                         int r = tl / 3, d = tl - (tl / 3) * 3;
-                        double rt_aa = rw[2*r] / aijkl;
+                        double rt_aa = rw[2*r] * r_aa;
                         double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
-                        double b10 = .5 / aij * (1. - rt_aij);
-                        double b01 = .5 / akl * (1. - rt_akl);
+                        double b10 = .5 * raij * (1. - rt_aij);
+                        double b01 = .5 * rakl * (1. - rt_akl);
                         double b00 = .5 * rt_aa;
                         double g[GS];
                         double pa = d == 0 ? PA[0] : (d == 1 ? PA[1] : PA[2]);
                         double pq = d == 0 ? PQ[0] : (d == 1 ? PQ[1] : PQ[2]);
+                        double qc = d == 0 ? QC[0] : (d == 1 ? QC[1] : QC[2]);
                         double rd = d == 0 ? sp.rij[0] : (d == 1 ? sp.rij[1] : sp.rij[2]);
                         double kd = d == 0 ? rkl[0] : (d == 1 ? rkl[1] : rkl[2]);
                         double g00 = d == 2 ? rw[2*r+1] * fac : 1.;
                         build_g<LI, LJ, LK, LL>(g, g00, pa - rt_aij * pq,
-                                                kd * al_akl + rt_akl * pq,
+                                                qc + rt_akl * pq,
                                                 b10, b01, b00, rd, kd);
                         double *dst = gbuf + (r * 3 + d) * GS;
 #pragma unroll
@@ -807,12 +854,20 @@ template <int LI, int LJ, int LK, int LL>
 static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
 {
     if constexpr (ClassTraits<LI, LJ, LK, LL>::LANE) {
-        const int kl_chunk = 1024;
-        int nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
+        // whole kl lists per work item unless that leaves the GPU idle
+        int nchunks = (4096 + a.npairs_ij - 1) / a.npairs_ij;
+        int maxchunks = (a.npairs_kl + 255) / 256;
+        if (nchunks > maxchunks) nchunks = maxchunks;
+        if (nchunks < 1) nchunks = 1;
+        const int kl_chunk = (a.npairs_kl + nchunks - 1) / nchunks;
+        nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-        hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(WAVE), 0,
-                           stream, a, kl_chunk);
+        constexpr int NFIJ = ncart(LI) + ncart(LJ);
+        size_t lds = (size_t)NFIJ * a.l_nao * sizeof(double);
+        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds <= 48 * 1024;
+        hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(WAVE),
+                           use_lds ? lds : 0, stream, a, kl_chunk, use_lds);
     } else {
         using S = TeamShape<LI, LJ, LK, LL>;
         const int kl_chunk = 256;
@@ -868,6 +923,32 @@ __global__ void ovlp_mask_kernel(int8_t *mask, const float *exps, const float *l
         }
     }
     mask[(size_t)ish * nbas + jsh] = keep;
+}
+
+__global__ void pair_data_kernel(double *out, const uint32_t *pairs, int npairs, int iprim,
+                                 int jprim, const int32_t *bas, const double *env, int nbas)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int np = iprim * jprim;
+    if (t >= npairs * np) return;
+    int p = t / np, ipjp = t - p * np;
+    int ip = ipjp / jprim, jp = ipjp - ip * jprim;
+    uint32_t bij = pairs[p];
+    int ish = bij / nbas, jsh = bij - (bij / nbas) * nbas;
+    const int32_t *bi = bas + ish * 8, *bj = bas + jsh * 8;
+    const double *ri = env + bi[7], *rj = env + bj[7];
+    double ai = env[bi[5] + ip], aj = env[bj[5] + jp];
+    double a = ai + aj, ra = 1. / a;
+    double rr = 0.;
+    double *o = out + (size_t)t * 6;
+    for (int d = 0; d < 3; ++d) {
+        double dr = rj[d] - ri[d];
+        rr += dr * dr;
+        o[2 + d] = ri[d] + dr * aj * ra;
+    }
+    o[0] = a;
+    o[1] = ra;
+    o[5] = sqrt(PI_FAC) * env[bi[6] + ip] * env[bj[6] + jp] * exp(-ai * aj * ra * rr) * ra;
 }
 
 // max over the Cartesian diagonal (ab|ab) of a shell pair; by the
@@ -1027,7 +1108,7 @@ int jk_build(const JKBuildArgs *args, void *stream)
     }
     if (a.npairs_ij <= 0 || a.npairs_kl <= 0) return 0;
     if (!a.bas || !a.env || !a.ao_loc || !a.pairs_ij || !a.pairs_kl || !a.q_ij ||
-        !a.q_kl || !a.dm_cond || !a.dm || (!a.vj && !a.vk) || a.n_dm <= 0 ||
+        !a.q_kl || !a.pdata_ij || !a.pdata_kl || !a.dm_cond || !a.dm || (!a.vj && !a.vk) || a.n_dm <= 0 ||
         a.nbas <= 0 || a.nao <= 0) {
         set_err("jk_build: null pointer or empty dimension");
         return 3;
@@ -1044,6 +1125,23 @@ int jk_build(const JKBuildArgs *args, void *stream)
                  hipGetErrorString(err));
         return 4;
     }
+    return 0;
+}
+
+int jk_pair_data(double *out, const uint32_t *pairs, int npairs, int iprim,
+                 int jprim, const int32_t *bas, const double *env, int nbas,
+                 void *stream)
+{
+    g_err[0] = 0;
+    if (npairs <= 0) return 0;
+    if (!out || !pairs || !bas || !env || iprim <= 0 || jprim <= 0 || nbas <= 0) {
+        set_err("jk_pair_data: bad arguments"); return 3;
+    }
+    long n = (long)npairs * iprim * jprim;
+    hipLaunchKernelGGL(pair_data_kernel, dim3((n + 127) / 128), dim3(128), 0,
+                       (hipStream_t)stream, out, pairs, npairs, iprim, jprim, bas, env, nbas);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) { set_err("jk_pair_data: %s", hipGetErrorString(err)); return 4; }
     return 0;
 }
 

@@ -14,7 +14,7 @@ _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, 'libjkhip.so')
 TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
 
-JK_ABI_VERSION = 1
+JK_ABI_VERSION = 3
 JK_RYS_NMAX = 11
 JK_LMAX = 3
 
@@ -43,9 +43,12 @@ class JKBuildArgs(ctypes.Structure):
         ('li', ctypes.c_int), ('lj', ctypes.c_int), ('lk', ctypes.c_int), ('ll', ctypes.c_int),
         ('iprim', ctypes.c_int), ('jprim', ctypes.c_int),
         ('kprim', ctypes.c_int), ('lprim', ctypes.c_int),
+        ('k_ao0', ctypes.c_int), ('k_nao', ctypes.c_int),
+        ('l_ao0', ctypes.c_int), ('l_nao', ctypes.c_int),
         ('npairs_ij', ctypes.c_int), ('npairs_kl', ctypes.c_int),
         ('pairs_ij', c_void_p), ('pairs_kl', c_void_p),
         ('q_ij', c_void_p), ('q_kl', c_void_p),
+        ('pdata_ij', c_void_p), ('pdata_kl', c_void_p),
         ('dm_cond', c_void_p),
         ('cutoff', ctypes.c_float),
         ('dm_penalty', ctypes.c_float),
@@ -58,7 +61,7 @@ class JKBuildArgs(ctypes.Structure):
     ]
 
 
-EXPORTS = ('jk_build', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
+EXPORTS = ('jk_build', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
            'jk_last_error', 'jk_abi_version')
 
 _lib = None
@@ -74,7 +77,8 @@ def load():
                            'or make -C gpu4pyscf_amd/csrc')
     lib = ctypes.CDLL(LIB_PATH)
     lib.jk_last_error.restype = ctypes.c_char_p
-    for name in ('jk_build', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond', 'jk_abi_version'):
+    for name in ('jk_build', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
+                 'jk_abi_version'):
         getattr(lib, name).restype = ctypes.c_int
     lib.jk_build.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p]
     lib.jk_pair_qcond.argtypes = [c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p,

@@ -151,8 +151,9 @@ class _VHFOpt:
             q_all = q_d.cpu().numpy()
         split = np.arange(math.log(self.direct_scf_tol), 2., Q_COND_MARGIN)
         cache = {}
-        dev_pairs, dev_q = {}, {}
+        dev_pairs, dev_q, dev_pd = {}, {}, {}
         pos = 0
+        uniq = mol.uniq_l_ctr
         for (i, j), p in zip(keys, plist):
             q = q_all[pos:pos+len(p)]
             pos += len(p)
@@ -160,11 +161,20 @@ class _VHFOpt:
             pij = p[idx].astype(np.uint32)
             qij = q[idx].astype(np.float32)
             cache[i, j] = (pij, qij, qij)
-            dev_pairs[i, j] = torch.as_tensor(pij.astype(np.int64), device=dev).to(torch.int32)
+            pt = torch.as_tensor(pij.astype(np.int64), device=dev).to(torch.int32)
+            dev_pairs[i, j] = pt
             dev_q[i, j] = torch.as_tensor(qij, device=dev)
+            ip, jp = int(uniq[i, 1]), int(uniq[j, 1])
+            pd = torch.empty((max(len(pij), 1), ip * jp, 6), dtype=torch.float64, device=dev)
+            clib.check(lib.jk_pair_data(_ptr(pd), _ptr(pt), ctypes.c_int(len(pij)),
+                                        ctypes.c_int(ip), ctypes.c_int(jp), _ptr(d['bas']),
+                                        _ptr(d['env']), ctypes.c_int(nbas), stream),
+                       'jk_pair_data')
+            dev_pd[i, j] = pd
         self.bas_pair_cache = cache
         d['pairs'] = dev_pairs
         d['q'] = dev_q
+        d['pdata'] = dev_pd
         self._dev = d
         self.device = dev
         self.timings['build'] = time.perf_counter() - t0
@@ -258,7 +268,12 @@ class _VHFOpt:
         args.vk = vk.data_ptr() if with_k else None
         args.rys = self._rys
         uniq = mol.uniq_l_ctr
+        goff = mol.group_offsets
+        g_ao0 = mol.ao_loc[goff[:-1]]
+        g_nao = mol.ao_loc[goff[1:]] - g_ao0
         for t, (i, j, k, l) in enumerate(tasks):
+            args.k_ao0, args.k_nao = int(g_ao0[k]), int(g_nao[k])
+            args.l_ao0, args.l_nao = int(g_ao0[l]), int(g_nao[l])
             if counter is not None:
                 args.n_quartets = counter.data_ptr() + 8 * t
             pij, qij = d['pairs'][i, j], d['q'][i, j]
@@ -271,6 +286,8 @@ class _VHFOpt:
             args.pairs_kl = pkl.data_ptr()
             args.q_ij = qij.data_ptr()
             args.q_kl = qkl.data_ptr()
+            args.pdata_ij = d['pdata'][i, j].data_ptr()
+            args.pdata_kl = d['pdata'][k, l].data_ptr()
             if time_classes:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  1
+#define JK_ABI_VERSION  3
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         3       /* max angular momentum on the GPU path */
 
@@ -54,10 +54,13 @@ typedef struct {
     /* class: all shells of a group share l and nprim */
     int li, lj, lk, ll;
     int iprim, jprim, kprim, lprim;
+    /* AO columns [k_ao0, k_ao0 + k_nao) of the k group, same for l */
+    int k_ao0, k_nao, l_ao0, l_nao;
     /* screened pair lists of the two group pairs, in descending-q bins */
     int npairs_ij, npairs_kl;
     const uint32_t *pairs_ij, *pairs_kl;    /* device */
     const float *q_ij, *q_kl;               /* device, log sqrt max|(ij|ij)| */
+    const double *pdata_ij, *pdata_kl;      /* device, jk_pair_data() records */
     const float *dm_cond;                   /* device, nbas x nbas */
     float cutoff;                           /* log(direct_scf_tol) */
     float dm_penalty;                       /* max(dm_cond) */
@@ -72,6 +75,17 @@ typedef struct {
 /* Accumulate one class task into vj/vk (before the hermi post-processing of
  * gpu4pyscf/scf/jk.py:594-616).  stream: hipStream_t or NULL. */
 int jk_build(const JKBuildArgs *args, void *stream);
+
+/* Primitive-pair records of a pair list, geometry-only data precomputed once
+ * per molecule: out[(p*iprim*jprim + ip*jprim + jp)*6 + (0..5)] =
+ * (a, 1/a, Px, Py, Pz, K) with a = ai + aj, P = (ai Ri + aj Rj)/a and
+ * K = sqrt(2 pi^2.5) ci cj exp(-ai aj/a |Ri - Rj|^2) / a, so that the ERI
+ * prefactor of a primitive quartet is K_ij K_kl / sqrt(a_ij + a_kl).
+ * No counterpart in the reference (it recomputes these in every kernel,
+ * gpu4pyscf/lib/gvhf-rys/rys_contract_jk.cu:146-160). */
+int jk_pair_data(double *out, const uint32_t *pairs, int npairs, int iprim,
+                 int jprim, const int32_t *bas, const double *env, int nbas,
+                 void *stream);
 
 /* float32 shell-pair overlap mask (lower triangle), replaces
  * PBCovlp_mask_estimation (gpu4pyscf/lib/pbc/overlap.cu:2378-2387) as used by
