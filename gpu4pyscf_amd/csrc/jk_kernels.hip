@@ -27,12 +27,18 @@
 #define Q_COND_MARGIN   4.f
 #define WAVE            64
 #define RYS_DEG_MAX     15
+#define LANE_WAVES      4       // wavefronts per block of the per-lane kernel
 
 static thread_local char g_err[512];
 
 static void set_err(const char *fmt, const char *a = "", int x = 0, int y = 0)
 {
     snprintf(g_err, sizeof(g_err), fmt, a, x, y);
+}
+
+void jk_set_error(const char *msg)
+{
+    snprintf(g_err, sizeof(g_err), "%s", msg);
 }
 
 // ---------------------------------------------------------------- helpers
@@ -61,6 +67,14 @@ __device__ __forceinline__ double wave_sum(double v)
     for (int off = 32; off > 0; off >>= 1)
         v += __shfl_xor(v, off, WAVE);
     return v;
+}
+
+// Orders LDS accesses of the lanes of one wavefront (no block barrier).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Inclusive segmented sum over lanes with equal, contiguous segment ids:
@@ -328,13 +342,17 @@ __device__ __forceinline__ void eri_lane(double *gout, const double *pij, int ni
 }
 
 template <int LI, int LJ, int LK, int LL>
-__global__ __launch_bounds__(WAVE)
+__global__ __launch_bounds__(LANE_WAVES * WAVE)
 void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
 {
     constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
     constexpr int NF = NFI * NFJ * NFK * NFL;
-    __shared__ uint32_t queue[2 * WAVE];
-    const int lane = threadIdx.x;
+    // LANE_WAVES wavefronts share one work item (and its LDS K rows); each
+    // wavefront scans every LANE_WAVES-th batch of 64 kl candidates
+    __shared__ uint32_t queue_all[LANE_WAVES][2 * WAVE];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    uint32_t *queue = queue_all[wid];
     const int nbas = a.nbas, nao = a.nao;
     WorkItem wi = work_item(a.npairs_kl, kl_chunk);
     const uint32_t bas_ij = a.pairs_ij[wi.pij];
@@ -348,14 +366,14 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     double *kacc = use_lds ? kacc_dyn : nullptr;
     const int nacc = (NFI + NFJ) * a.l_nao;
     if (kacc) {
-        for (int n = lane; n < nacc; n += WAVE) kacc[n] = 0.;
+        for (int n = threadIdx.x; n < nacc; n += LANE_WAVES * WAVE) kacc[n] = 0.;
         __syncthreads();
     }
     int qn = 0;
     unsigned long long nsurv = 0;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     bool stop = false;
-    for (int base = wi.kl0; base < wi.kl1 || qn > 0; base += WAVE) {
+    for (int base = wi.kl0 + wid * WAVE; base < wi.kl1 || qn > 0; base += LANE_WAVES * WAVE) {
         if (base < wi.kl1 && !stop) {
             int pkl = base + lane;
             bool keep = false, fail_margin = false;
@@ -370,9 +388,9 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
             qn += __popcll(m);
             nsurv += __popcll(m);
             stop = __ballot(fail_margin) != 0;
-            __syncthreads();
+            wave_sync();
         }
-        bool flush = (base + WAVE >= wi.kl1) || stop;
+        bool flush = (base + LANE_WAVES * WAVE >= wi.kl1) || stop;
         if (qn == 0) {
             // nothing queued: the queue holds no valid entry to read
             if (flush) break;
@@ -523,17 +541,17 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
             }
         }
         // shift the remainder of the queue to the front
-        __syncthreads();
+        wave_sync();
         uint32_t rest = (nb + lane < qn) ? queue[nb + lane] : 0u;
-        __syncthreads();
+        wave_sync();
         if (nb + lane < qn) queue[lane] = rest;
         qn -= nb;
-        __syncthreads();
+        wave_sync();
         if (flush && qn == 0) break;
     }
     if (kacc) {
         __syncthreads();
-        for (int n = lane; n < nacc; n += WAVE) {
+        for (int n = threadIdx.x; n < nacc; n += LANE_WAVES * WAVE) {
             double v = kacc[n];
             if (v != 0.) {
                 int r = n / a.l_nao, c = n - r * a.l_nao;
@@ -865,8 +883,8 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
         constexpr int NFIJ = ncart(LI) + ncart(LJ);
         size_t lds = (size_t)NFIJ * a.l_nao * sizeof(double);
-        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds <= 48 * 1024;
-        hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(WAVE),
+        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds <= 64 * 1024;
+        hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(LANE_WAVES * WAVE),
                            use_lds ? lds : 0, stream, a, kl_chunk, use_lds);
     } else {
         using S = TeamShape<LI, LJ, LK, LL>;

@@ -62,7 +62,7 @@ class JKBuildArgs(ctypes.Structure):
 
 
 EXPORTS = ('jk_build', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
-           'jk_last_error', 'jk_abi_version')
+           'jk_int1e', 'jk_last_error', 'jk_abi_version')
 
 _lib = None
 
@@ -83,6 +83,9 @@ def load():
     lib.jk_build.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p]
     lib.jk_pair_qcond.argtypes = [c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p,
                                   ctypes.c_int, JKRysTable, c_void_p]
+    lib.jk_int1e.restype = ctypes.c_int
+    lib.jk_int1e.argtypes = [c_void_p] * 6 + [ctypes.c_int, ctypes.c_int, c_void_p, c_void_p,
+                                               ctypes.c_int, JKRysTable, c_void_p]
     if lib.jk_abi_version() != JK_ABI_VERSION:
         raise RuntimeError('libjkhip.so ABI version mismatch; rebuild it')
     _lib = lib

@@ -1,0 +1,239 @@
+'''Restricted Hartree-Fock driver on the MI355X J/K engine.
+
+Follows the structure of gpu4pyscf/scf/hf.py:206-391 (``_kernel``): the
+Fock matrix is built incrementally on the density change (get_veff,
+gpu4pyscf/scf/hf.py:91-109), CDIIS extrapolation starts at cycle 1 with 8
+vectors (gpu4pyscf/scf/diis.py:41-127), the generalised eigenproblem is
+solved in the canonically orthogonalised basis with overlap eigenvalues below
+1e-6 removed (check_linear_dependency, gpu4pyscf/scf/hf.py:625-641), and
+convergence requires |dE| < conv_tol and |g| < sqrt(conv_tol)
+(gpu4pyscf/scf/hf.py:309-312).  Everything is FP64 on the device.
+
+The initial guess is 'atom' (superposition of spherically averaged atomic
+RHF densities, the analogue of init_guess_by_atom, gpu4pyscf/scf/hf.py:
+561-607), '1e' (core Hamiltonian), a user density, or a callable (used by the
+learned guess).  The ANO-based 'minao' guess of PySCF needs basis data that is
+not part of this project.
+'''
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from gpu4pyscf_amd.lib import clib
+from gpu4pyscf_amd.scf import jk as jk_mod
+
+LINDEP_THRESHOLD = 1e-6
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def int1e(mol, vhfopt):
+    '''(S, T, V) in the original AO basis of ``mol`` as device tensors.'''
+    torch = _torch()
+    lib = clib.load()
+    if vhfopt._dev is None:
+        vhfopt.build()
+    sm = vhfopt.sorted_mol
+    d = vhfopt._dev
+    dev = vhfopt.device
+    n = sm.nao
+    S = torch.empty((n, n), dtype=torch.float64, device=dev)
+    T = torch.empty_like(S)
+    V = torch.empty_like(S)
+    coords = torch.as_tensor(np.asarray(mol.atom_coords(), dtype=np.float64), device=dev)
+    charges = torch.as_tensor(np.asarray(mol.atom_charges(), dtype=np.float64), device=dev)
+    st = lib.jk_int1e(S.data_ptr(), T.data_ptr(), V.data_ptr(), d['bas'].data_ptr(),
+                      d['env'].data_ptr(), d['ao_loc'].data_ptr(), sm.nbas, n,
+                      coords.data_ptr(), charges.data_ptr(), mol.natm, vhfopt._rys,
+                      jk_mod._stream_ptr())
+    clib.check(st, 'jk_int1e')
+    C = d['coeff']
+    return C.T @ S @ C, C.T @ T @ C, C.T @ V @ C
+
+
+class CDIIS:
+    '''Commutator DIIS on the error FDS - SDF in the orthonormal basis.'''
+
+    def __init__(self, space=8):
+        self.space = space
+        self.fs = []
+        self.es = []
+
+    def update(self, f, e):
+        torch = _torch()
+        self.fs.append(f)
+        self.es.append(e)
+        if len(self.fs) > self.space:
+            self.fs.pop(0)
+            self.es.pop(0)
+        n = len(self.fs)
+        B = torch.empty((n + 1, n + 1), dtype=torch.float64, device=f.device)
+        E = torch.stack([x.reshape(-1) for x in self.es])
+        B[:n, :n] = E @ E.T
+        B[n, :n] = B[:n, n] = -1.
+        B[n, n] = 0.
+        rhs = torch.zeros(n + 1, dtype=torch.float64, device=f.device)
+        rhs[n] = -1.
+        try:
+            c = torch.linalg.solve(B, rhs)
+        except RuntimeError:
+            c = torch.linalg.lstsq(B.cpu(), rhs.cpu()).solution.to(f.device)
+        return sum(c[i] * self.fs[i] for i in range(n))
+
+
+class RHF:
+    '''Closed-shell RHF (gpu4pyscf.scf.RHF surface: kernel, e_tot, converged,
+    mo_energy, mo_coeff, mo_occ, make_rdm1, get_jk, get_veff, cycles).'''
+
+    conv_tol = 1e-9
+    conv_tol_grad = None
+    max_cycle = 50
+    diis_space = 8
+    diis_start_cycle = 1
+    direct_scf_tol = 1e-13
+    init_guess = 'atom'
+
+    def __init__(self, mol):
+        self.mol = mol
+        self.verbose = getattr(mol, 'verbose', 0)
+        self._opt = None
+        self.converged = False
+        self.e_tot = 0.
+        self.cycles = 0
+        self.mo_energy = self.mo_coeff = self.mo_occ = None
+        self.jk_builds = 0
+        self.timings = {}
+        self.history = []
+        self.jk_hook = None
+
+    # -- integrals -------------------------------------------------------
+    @property
+    def opt(self):
+        if self._opt is None:
+            self._opt = jk_mod._VHFOpt(self.mol, self.direct_scf_tol).build()
+        return self._opt
+
+    def get_hcore_ovlp(self):
+        S, T, V = int1e(self.mol, self.opt)
+        return T + V, S
+
+    def get_jk(self, mol=None, dm=None, hermi=1, with_j=True, with_k=True):
+        '''J and K of ``dm`` (device tensor in the original AO basis).'''
+        self.jk_builds += 1
+        if self.jk_hook is not None:
+            return self.jk_hook(dm, hermi)
+        opt = self.opt
+        dms = opt.apply_coeff_C_mat_CT(dm.reshape(-1, dm.shape[-1], dm.shape[-1]))
+        vj, vk = opt.get_jk(dms, hermi, with_j=with_j, with_k=with_k)
+        shape = dm.shape
+        vj = opt.apply_coeff_CT_mat_C(vj).reshape(shape) if with_j else None
+        vk = opt.apply_coeff_CT_mat_C(vk).reshape(shape) if with_k else None
+        return vj, vk
+
+    def get_veff(self, dm, dm_last=None, vhf_last=None):
+        '''J - K/2 built on the density change (gpu4pyscf/scf/hf.py:91-109).'''
+        if dm_last is None or vhf_last is None:
+            vj, vk = self.get_jk(self.mol, dm, hermi=1)
+            return vj - .5 * vk
+        vj, vk = self.get_jk(self.mol, dm - dm_last, hermi=1)
+        return vhf_last + vj - .5 * vk
+
+    # -- linear algebra ---------------------------------------------------
+    def _orth(self, S):
+        torch = _torch()
+        w, U = torch.linalg.eigh(S)
+        keep = w > LINDEP_THRESHOLD
+        return U[:, keep] / torch.sqrt(w[keep])
+
+    def eig(self, F, X):
+        torch = _torch()
+        e, c = torch.linalg.eigh(X.T @ F @ X)
+        return e, X @ c
+
+    def get_occ(self, mo_energy):
+        torch = _torch()
+        nocc = self.mol.nelectron // 2
+        occ = torch.zeros_like(mo_energy)
+        idx = torch.argsort(mo_energy)
+        occ[idx[:nocc]] = 2.
+        return occ
+
+    @staticmethod
+    def make_rdm1(mo_coeff, mo_occ):
+        return (mo_coeff * mo_occ) @ mo_coeff.T
+
+    # -- initial guess ----------------------------------------------------
+    def get_init_guess(self, h1e, S, X, key=None):
+        torch = _torch()
+        key = self.init_guess if key is None else key
+        if callable(key):
+            dm = key(self.mol)
+            return torch.as_tensor(dm, dtype=torch.float64, device=S.device)
+        if not isinstance(key, str):
+            return torch.as_tensor(key, dtype=torch.float64, device=S.device)
+        if key == '1e':
+            e, c = self.eig(h1e, X)
+            return self.make_rdm1(c, self.get_occ(e))
+        if key == 'atom':
+            from gpu4pyscf_amd.scf.atom_guess import atom_guess
+            return torch.as_tensor(atom_guess(self.mol), dtype=torch.float64, device=S.device)
+        raise ValueError(f'unknown init_guess {key!r}')
+
+    # -- driver -------------------------------------------------------------
+    def kernel(self, dm0=None):
+        torch = _torch()
+        t_start = time.perf_counter()
+        mol = self.mol
+        e_nuc = mol.energy_nuc()
+        h1e, S = self.get_hcore_ovlp()
+        X = self._orth(S)
+        conv_tol_grad = self.conv_tol_grad or math.sqrt(self.conv_tol)
+        dm = self.get_init_guess(h1e, S, X, dm0 if dm0 is not None else None)
+        t_guess = time.perf_counter()
+        vhf = self.get_veff(dm)
+        e_tot = float(torch.sum(dm * (h1e + .5 * vhf))) + e_nuc
+        diis = CDIIS(self.diis_space)
+        self.converged = False
+        self.history = [e_tot]
+        cycle = 0
+        for cycle in range(self.max_cycle):
+            dm_last = dm
+            last_e = e_tot
+            fock = h1e + vhf
+            if cycle >= self.diis_start_cycle:
+                sdf = S @ dm @ fock
+                err = X.T @ (sdf.T - sdf) @ X
+                fock = diis.update(fock, err)
+            mo_energy, mo_coeff = self.eig(fock, X)
+            mo_occ = self.get_occ(mo_energy)
+            dm = self.make_rdm1(mo_coeff, mo_occ)
+            vhf = self.get_veff(dm, dm_last, vhf)
+            e_tot = float(torch.sum(dm * (h1e + .5 * vhf))) + e_nuc
+            fock = h1e + vhf
+            # orbital gradient: occupied-virtual block of the MO Fock matrix
+            occ = mo_occ > 0
+            f_mo = mo_coeff.T @ fock @ mo_coeff
+            g = f_mo[~occ][:, occ]
+            norm_gorb = float(torch.linalg.norm(g)) * 2.
+            self.history.append(e_tot)
+            if self.verbose:
+                print(f'cycle {cycle+1}: E = {e_tot:.12f} dE = {e_tot-last_e:.3e} '
+                      f'|g| = {norm_gorb:.3e}', flush=True)
+            if abs(e_tot - last_e) < self.conv_tol and norm_gorb < conv_tol_grad:
+                self.converged = True
+                break
+        self.cycles = cycle + 1
+        self.e_tot = e_tot
+        self.mo_energy, self.mo_coeff, self.mo_occ = mo_energy, mo_coeff, mo_occ
+        self.dm = dm
+        torch.cuda.synchronize()
+        self.timings = dict(guess=t_guess - t_start, total=time.perf_counter() - t_start)
+        return e_tot
+
+    scf = kernel

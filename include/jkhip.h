@@ -112,6 +112,15 @@ int jk_dm_cond(float *dm_cond, double *work, const double *dm, int n_dm,
                int nao, const int32_t *ao_loc, int nbas, int hermi,
                void *stream);
 
+/* One-electron integrals on the sorted Cartesian basis (nao x nao, written
+ * fully): overlap S, kinetic T and nuclear attraction
+ * V = -sum_A Z_A <i|1/|r-R_A||j> of point charges.  coords: natm x 3 Bohr,
+ * charges: natm (device).  The reference computes these for its SCF driver
+ * (get_hcore / get_ovlp, gpu4pyscf/scf/hf.py:138-155, 880-883). */
+int jk_int1e(double *S, double *T, double *V, const int32_t *bas, const double *env,
+             const int32_t *ao_loc, int nbas, int nao, const double *coords,
+             const double *charges, int natm, JKRysTable rys, void *stream);
+
 /* Last error message of this thread ("" if none). */
 const char *jk_last_error(void);
 
