@@ -1,0 +1,144 @@
+'''CPU tests of the host side: the C-ABI library loads and exports every
+declared symbol (no GPU calls), the screening bookkeeping equals the
+oracle's restatement, the basis layer's invariants and the workload
+generators.'''
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, 'include', 'jkhip.h')).read()
+    return sorted(set(re.findall(r'^\s*(?:int|const char \*)\s*(jk_\w+)\s*\(', text, re.M)))
+
+
+def test_library_exports_header_symbols():
+    from gpu4pyscf_amd.lib import clib
+    lib = clib.load()
+    syms = _declared_symbols()
+    assert 'jk_build' in syms and 'jk_int1e' in syms
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(clib.EXPORTS)
+    assert lib.jk_abi_version() == clib.JK_ABI_VERSION
+
+
+def test_struct_layout_matches_header():
+    from gpu4pyscf_amd.lib import clib
+    # JKRysTable: ptr + 11 int64 + 11 double + 2 int64 + int (+pad)
+    assert ctypes.sizeof(clib.JKRysTable) == 8 + 88 + 88 + 16 + 8
+    assert clib.JKBuildArgs.rys.offset % 8 == 0
+
+
+def test_jk_build_rejects_bad_arguments():
+    '''Argument validation runs on the host before any launch.'''
+    from gpu4pyscf_amd.lib import clib
+    lib = clib.load()
+    a = clib.JKBuildArgs()
+    a.abi_version = clib.JK_ABI_VERSION + 7
+    assert lib.jk_build(ctypes.byref(a), None) == 1
+    a.abi_version = clib.JK_ABI_VERSION
+    a.li, a.lj, a.lk, a.ll = 5, 0, 0, 0
+    assert lib.jk_build(ctypes.byref(a), None) == 2
+    a.li = 1
+    a.npairs_ij = a.npairs_kl = 3
+    assert lib.jk_build(ctypes.byref(a), None) == 3
+    assert b'null' in lib.jk_last_error()
+
+
+def test_tile_pairs_and_bins_match_oracle():
+    from gpu4pyscf_amd.scf import jk
+    from oracle import oracle as O
+    for (i0, i1, j0, j1, nb) in [(0, 30, 0, 30, 40), (5, 31, 2, 9, 40), (0, 13, 0, 25, 25)]:
+        assert np.array_equal(jk.tile_pairs(i0, i1, j0, j1, nb, 12),
+                              O.tile_pairs(i0, i1, j0, j1, nb, 12))
+    rng = np.random.default_rng(0)
+    q = rng.uniform(-40, 3, 500).astype(np.float32)
+    split = np.arange(math.log(1e-13), 2., 4.)
+    assert np.array_equal(jk.group_by_split_points(q, split), O.group_by_split_points(q, split))
+
+
+def test_sorted_basis_invariants():
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    mol = M(atom='O 0 0 0; H 0 1 0; C 1 0 0; H 0 0 1', basis='def2-tzvp')
+    sm = SortedGTO(mol)
+    l = sm._bas[:, 1]
+    np_ = sm._bas[:, 2]
+    key = l * 100 - np_
+    assert np.all(np.diff(key) >= 0)          # sorted by (l, -nprim)
+    assert sm.coeff.shape == (sm.nao, mol.nao)
+    # every original AO maps to at least one sorted Cartesian AO
+    assert np.all(abs(sm.coeff).sum(axis=0) > 0)
+
+
+@pytest.mark.parametrize('shape,spacing,basis,nao', [
+    ((2, 2, 2), 3.0, 'def2-svp', 192),
+    ((4, 4, 2), 3.0, 'def2-tzvp', 1376),
+    ((4, 4, 4), 3.1, 'def2-tzvp', 2752),
+])
+def test_water_cluster_sizes(shape, spacing, basis, nao):
+    from gpu4pyscf_amd.gto import geometry as G
+    from gpu4pyscf_amd.gto.mole import M
+    atoms = G.water_cluster(shape, spacing, seed=0)
+    assert len(atoms) == 3 * np.prod(shape)
+    for k in range(0, len(atoms), 3):
+        o, h1, h2 = atoms[k][1], atoms[k+1][1], atoms[k+2][1]
+        assert abs(np.linalg.norm(h1 - o) - 0.9572) < 1e-12
+        assert abs(np.linalg.norm(h2 - o) - 0.9572) < 1e-12
+        c = np.dot(h1 - o, h2 - o) / 0.9572**2
+        assert abs(np.degrees(np.arccos(c)) - 104.52) < 1e-9
+    mol = M(atom=G.to_string(atoms), basis=basis)
+    assert mol.nao == nao
+    # deterministic
+    assert G.to_string(atoms) == G.to_string(G.water_cluster(shape, spacing, seed=0))
+
+
+def test_alkane_geometry():
+    from gpu4pyscf_amd.gto import geometry as G
+    atoms = G.alkane(30, seed=0, sigma=0.)
+    sym = [a[0] for a in atoms]
+    assert sym.count('C') == 30 and sym.count('H') == 62
+    C = np.array([a[1] for a in atoms if a[0] == 'C'])
+    H = np.array([a[1] for a in atoms if a[0] == 'H'])
+    dcc = np.linalg.norm(C[1:] - C[:-1], axis=1)
+    assert abs(dcc - 1.54).max() < 1e-12
+    # every H is 1.09 from its nearest C and carbons are tetrahedral
+    dch = np.min(np.linalg.norm(H[:, None] - C[None], axis=2), axis=1)
+    assert abs(dch - 1.09).max() < 1e-12
+    v1, v2 = C[0] - C[1], C[2] - C[1]
+    ang = np.degrees(np.arccos(np.dot(v1, v2) / np.linalg.norm(v1) / np.linalg.norm(v2)))
+    assert abs(ang - 109.4712206) < 1e-6
+    from gpu4pyscf_amd.gto.mole import M
+    mol = M(atom=G.to_string(G.alkane(30, seed=0)), basis='def2-tzvp')
+    assert mol.nao == 1302
+
+
+def test_cart2sph_pins():
+    '''Coefficients quoted in gpu4pyscf/lib/cupy_helper/cart2sph.cu:38-95.'''
+    from gpu4pyscf_amd.gto.cart2sph import c2s
+    assert abs(c2s(2)[0, 1] - 1.092548430592079070) < 1e-15
+    assert abs(c2s(2)[2, 5] - 0.630783130505040012) < 1e-15
+    assert abs(c2s(3)[3, 9] - 0.746352665180230782) < 1e-15
+    assert abs(c2s(3)[6, 3] + 1.770130769779930530) < 1e-15
+    assert abs(c2s(4)[4, 14] - 0.846284375321634430) < 1e-15
+    assert abs(c2s(4)[6, 10] - 0.473087347878780009) < 1e-15
+    # orthonormal real harmonics: integrate products over the sphere
+    from math import factorial
+    def sphere_moment(a, b, c):
+        if a % 2 or b % 2 or c % 2:
+            return 0.
+        from scipy.special import gamma
+        return 2 * gamma((a+1)/2) * gamma((b+1)/2) * gamma((c+1)/2) / gamma((a+b+c+3)/2)
+    from gpu4pyscf_amd.gto.cart2sph import cart_components
+    for l in (2, 3, 4, 5):
+        comp = cart_components(l)
+        M_ = np.array([[sphere_moment(*(np.add(p, q))) for q in comp] for p in comp])
+        G_ = c2s(l) @ M_ @ c2s(l).T
+        assert abs(G_ - np.eye(2*l+1)).max() < 1e-12

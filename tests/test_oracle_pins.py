@@ -1,0 +1,150 @@
+'''Pins of the CPU oracle beyond the reference's golden vectors: closed
+forms, an independent brute-force ERI algorithm, and ground-truth Rys
+quadratures.'''
+import math
+import numpy as np
+import pytest
+import mpmath as mp
+
+from oracle import oracle as O
+
+
+def _boys(m, x):
+    mp.mp.dps = 30
+    x = mp.mpf(x)
+    if x == 0:
+        return mp.mpf(1) / (2 * m + 1)
+    return mp.gammainc(m + mp.mpf(1) / 2, 0, x) / (2 * x**(m + mp.mpf(1) / 2))
+
+
+@pytest.mark.parametrize('n', [1, 2, 4, 7, 9, 11])
+@pytest.mark.parametrize('x', [0., 1e-7, .3, 2.5, 11., 37., 80., 250.])
+def test_oracle_rys_moments(n, x):
+    u, w = O.rys_roots(n, x)
+    for m in range(2 * n):
+        ref = _boys(m, x)
+        val = sum(w[i] * u[i]**m for i in range(n))
+        assert abs(val - float(ref)) <= 2e-14 * float(ref)
+
+
+def test_rys_limits():
+    # x -> 0: nodes are squared positive roots of the Legendre P_2n
+    u, w = O.rys_roots(1, 0.)
+    assert abs(u[0] - 1. / 3) < 1e-15 and abs(w[0] - 1.) < 1e-15
+    s, ws = np.polynomial.legendre.leggauss(6)
+    ref = np.sort(s[s > 0]**2)
+    u, w = O.rys_roots(3, 0.)
+    assert abs(u - ref).max() < 1e-14
+    # x -> inf: squared positive Hermite roots / x
+    x = 400.
+    s, ws = np.polynomial.hermite.hermgauss(8)
+    u, w = O.rys_roots(4, x)
+    assert abs(u * x - np.sort(s[s > 0]**2)).max() < 1e-11
+
+
+def test_product_table_matches_oracle():
+    '''The product's Chebyshev Rys table against the oracle's roots.'''
+    t = np.load('gpu4pyscf_amd/csrc/rys_table.npz')
+    coef, off, xmax, D = t['coef'], t['off'], t['xmax'], int(t['degree'])
+    lr, lw = t['large_r'], t['large_w']
+    rng = np.random.default_rng(0)
+    for n in range(1, 12):
+        for x in np.concatenate([rng.random(8) * 110, [0., xmax[n-1] - 1e-9, xmax[n-1]]]):
+            if x >= xmax[n-1]:
+                b = n * (n - 1) // 2
+                r, w = lr[b:b+n] / x, lw[b:b+n] / math.sqrt(x)
+            else:
+                k = int(x)
+                c = coef[off[n-1] + k*2*n*(D+1): off[n-1] + (k+1)*2*n*(D+1)].reshape(2*n, D+1)
+                v = c @ np.cos(np.arange(D+1) * np.arccos(2 * (x - k) - 1))
+                r, w = v[0::2], v[1::2]
+            r0, w0 = O.rys_roots(n, x)
+            assert abs(r - r0).max() <= 5e-14 * abs(r0).max()
+            assert abs(w - w0).max() <= 5e-14 * abs(w0).max()
+
+
+def test_ssss_closed_form():
+    '''(ss|ss) of unnormalised-free s Gaussians: 2 pi^2.5/(pq sqrt(p+q)) K F0.'''
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    mol = M(atom='H 0 0 0; H 0 0 1.4; H 0.3 1.1 0; H 1 1 1', unit='B',
+            basis=[[0, [1.3, 1.]]])
+    sm = SortedGTO(mol)
+    a = 1.3
+    N = (2 * a / math.pi)**.75
+    R = mol.atom_coords()
+    eri = O.eri_shell(sm._bas, sm.env_kernel, 0, 1, 2, 3).ravel()[0]
+    p = q = 2 * a
+    P = (R[0] + R[1]) / 2; Q = (R[2] + R[3]) / 2
+    K = math.exp(-a / 2 * np.sum((R[0] - R[1])**2)) * math.exp(-a / 2 * np.sum((R[2] - R[3])**2))
+    T = p * q / (p + q) * np.sum((P - Q)**2)
+    F0 = .5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+    ref = N**4 * 2 * math.pi**2.5 / (p * q * math.sqrt(p + q)) * K * F0
+    assert abs(eri - ref) < 1e-14
+
+
+def test_rys_eri_vs_brute_force():
+    '''Oracle Rys shell quartets against the McMurchie-Davidson brute force.'''
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    from oracle import brute
+    mol = M(atom='O 0 0 0.2; H 0 1.3 -0.4; C 1.1 -0.2 0.5', unit='B', spin=1,
+            basis=[[0, [3.1, .5], [.9, .6]], [1, [1.7, 1.]], [2, [.8, 1.]], [3, [.6, 1.]]])
+    sm = SortedGTO(mol)
+    rng = np.random.default_rng(3)
+    for _ in range(12):
+        i, j, k, l = rng.integers(0, sm.nbas, 4)
+        a = O.eri_shell(sm._bas, sm.env_kernel, i, j, k, l)
+        b = brute.eri_shell(sm._bas, sm.env_kernel, i, j, k, l)
+        assert abs(a - b).max() < 1e-12 * max(1., abs(b).max())
+
+
+def test_eri_8fold_symmetry_and_jk_identities():
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    mol = M(atom='O 0 0 0.1174; H -0.757 0 -0.4696; H 0.757 0 -0.4696', basis='sto-3g')
+    sm = SortedGTO(mol)
+    eri = O.eri_full(sm)
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1)]:
+        assert abs(eri - eri.transpose(perm)).max() < 1e-14
+    # J/K of the oracle pipeline equal the einsum of the full tensor
+    C = sm.coeff
+    rng = np.random.default_rng(0)
+    d = rng.random((mol.nao, mol.nao)); d = d + d.T
+    vj, vk = O.get_jk(mol, d, hermi=1)
+    dc = C @ d @ C.T
+    rj = C.T @ np.einsum('ijkl,lk->ij', eri, dc) @ C
+    rk = C.T @ np.einsum('ijkl,jk->il', eri, dc) @ C
+    assert abs(vj - rj).max() < 1e-12 and abs(vk - rk).max() < 1e-12
+    # linearity and Tr(D J[D]) > 0
+    vj2, _ = O.get_jk(mol, 2 * d, hermi=1)
+    assert abs(vj2 - 2 * vj).max() < 1e-12
+    assert np.sum(d * vj) > 0
+
+
+def test_screening_drops_only_negligible():
+    '''Every quartet the predicate drops contributes below the cutoff bound.'''
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    mol = M(atom='O 0 0 0; H 0 0 1.8; O 0 0 9.; H 0 1.7 9.5', unit='B', basis='sto-3g')
+    sm = SortedGTO(mol)
+    cache = O.build_pairs(sm, 1e-13)
+    d = np.ones((1, sm.nao, sm.nao))
+    keys = set(O.survivors(sm, d, 1, cache).tolist())
+    n = sm.nbas
+    # q_cond is an upper bound of log sqrt|(ij|ij)| (Cauchy-Schwarz)
+    for (gi, gj), (p, q) in cache.items():
+        for pij, qq in zip(p, q):
+            i, j = divmod(int(pij), n)
+            blk = O.eri_shell(sm._bas, sm.env_kernel, i, j, i, j)
+            assert abs(math.log(abs(blk).max()) / 2 - qq) < 1e-5
+    assert len(keys) > 0
+
+
+def test_flops_model():
+    from gpu4pyscf_amd.scf.flops import quartet_flops, quartet_digest_bytes
+    # (ss|ss), one primitive: 1 root, 3 + 9 flops + 12 digestion
+    assert quartet_flops((0, 0, 0, 0), (1, 1, 1, 1)) == 24
+    # (pp|pp): n=3, Nf=81, G2=9
+    assert quartet_flops((1, 1, 1, 1), (1, 1, 1, 1)) == 3 * (243 + 81) + 12 * 81
+    assert quartet_digest_bytes((0, 0, 0, 0)) == 96
