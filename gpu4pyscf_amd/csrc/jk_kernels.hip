@@ -122,6 +122,33 @@ __device__ __forceinline__ void rys_roots(const JKRysTable &tab, double x,
     }
 }
 
+// Root `i` and its weight only (one lane per root).
+template <int N>
+__device__ __forceinline__ void rys_root_one(const JKRysTable &tab, double x, int i,
+                                             double &r, double &w)
+{
+    if (x >= tab.xmax[N-1]) {
+        double ix = 1. / x;
+        r = tab.coef[tab.large_r + N * (N - 1) / 2 + i] * ix;
+        w = tab.coef[tab.large_w + N * (N - 1) / 2 + i] * sqrt(ix);
+        return;
+    }
+    const int D1 = tab.degree + 1;
+    int k = (int)x;
+    double u = 2. * (x - k) - 1.;
+    double u2 = 2. * u;
+    const double *c = tab.coef + tab.off[N-1] + ((int64_t)k * (2 * N) + 2 * i) * D1;
+    double r1 = 0., r2 = 0., w1 = 0., w2 = 0.;
+    for (int j = D1 - 1; j > 0; --j) {
+        double t = fma(u2, r1, c[j] - r2);
+        r2 = r1; r1 = t;
+        double s = fma(u2, w1, c[D1 + j] - w2);
+        w2 = w1; w1 = s;
+    }
+    r = fma(u, r1, c[0] - r2);
+    w = fma(u, w1, c[D1] - w2);
+}
+
 // runtime-N variant (setup kernels)
 __device__ void rys_roots_rt(const JKRysTable &tab, int N, double x, double *r, double *w)
 {
@@ -152,34 +179,33 @@ __device__ void rys_roots_rt(const JKRysTable &tab, int N, double x, double *r, 
 // g[((i*(LJ+1) + j)*(LK+1) + k)*(LL+1) + l], by VRR in i, TRR in k and HRR
 // for j and l (the recursions of gpu4pyscf/lib/gvhf-rys/build_rys_gxyz.cuh:
 // 18-123, evaluated in registers).
-template <int LI, int LJ, int LK, int LL>
-__device__ __forceinline__ void build_g(double *g, double g00, double c0, double cp,
-                                        double b10, double b01, double b00,
-                                        double rij, double rkl)
+template <int LI, int LJ, int LK, int LL, typename T>
+__device__ __forceinline__ void build_g(T *g, T g00, T c0, T cp, T b10, T b01, T b00,
+                                        T rij, T rkl)
 {
     constexpr int LIJ = LI + LJ, LKL = LK + LL;
-    double G[LIJ+1][LKL+1];
+    T G[LIJ+1][LKL+1];
     G[0][0] = g00;
     if (LIJ > 0) G[1][0] = c0 * g00;
 #pragma unroll
     for (int a = 1; a < LIJ; ++a)
-        G[a+1][0] = c0 * G[a][0] + (a * b10) * G[a-1][0];
+        G[a+1][0] = c0 * G[a][0] + ((T)a * b10) * G[a-1][0];
 #pragma unroll
     for (int a = 0; a <= LIJ; ++a) {
         if (LKL > 0) {
-            double v = cp * G[a][0];
-            if (a > 0) v += (a * b00) * G[a-1][0];
+            T v = cp * G[a][0];
+            if (a > 0) v += ((T)a * b00) * G[a-1][0];
             G[a][1] = v;
         }
 #pragma unroll
         for (int c = 1; c < LKL; ++c) {
-            double v = cp * G[a][c] + (c * b01) * G[a][c-1];
-            if (a > 0) v += (a * b00) * G[a-1][c];
+            T v = cp * G[a][c] + ((T)c * b01) * G[a][c-1];
+            if (a > 0) v += ((T)a * b00) * G[a-1][c];
             G[a][c+1] = v;
         }
     }
     // bra HRR: H[a][b][c] = H[a+1][b-1][c] - rij H[a][b-1][c]
-    double H[LIJ+1][LJ+1][LKL+1];
+    T H[LIJ+1][LJ+1][LKL+1];
 #pragma unroll
     for (int c = 0; c <= LKL; ++c) {
 #pragma unroll
@@ -195,19 +221,19 @@ __device__ __forceinline__ void build_g(double *g, double g00, double c0, double
     for (int a = 0; a <= LI; ++a)
 #pragma unroll
     for (int b = 0; b <= LJ; ++b) {
-        double T[LKL+1][LL+1];
+        T W[LKL+1][LL+1];
 #pragma unroll
-        for (int c = 0; c <= LKL; ++c) T[c][0] = H[a][b][c];
+        for (int c = 0; c <= LKL; ++c) W[c][0] = H[a][b][c];
 #pragma unroll
         for (int e = 0; e < LL; ++e)
 #pragma unroll
             for (int c = 0; c < LKL - e; ++c)
-                T[c][e+1] = T[c+1][e] - rkl * T[c][e];
+                W[c][e+1] = W[c+1][e] - rkl * W[c][e];
 #pragma unroll
         for (int c = 0; c <= LK; ++c)
 #pragma unroll
             for (int e = 0; e <= LL; ++e)
-                g[((a*(LJ+1) + b)*(LK+1) + c)*(LL+1) + e] = T[c][e];
+                g[((a*(LJ+1) + b)*(LK+1) + c)*(LL+1) + e] = W[c][e];
     }
 }
 
@@ -249,6 +275,23 @@ __device__ __forceinline__ bool survives(float q_kl, float kl_cutoff, float dm_p
            dm_cond[jsh*nbas+lsh] > d_cutoff;
 }
 
+// Same predicate, also returning the largest of the six dm_cond values
+// (any(v > d_cutoff) == max(v) > d_cutoff).
+__device__ __forceinline__ bool survives_dmax(float q_kl, float kl_cutoff, float dm_penalty,
+                                              uint32_t bas_ij, uint32_t bas_kl, int ish, int jsh,
+                                              int nbas, const float *dm_cond, float &dmax)
+{
+    dmax = -1e30f;
+    if (!(q_kl + dm_penalty >= kl_cutoff && bas_ij >= bas_kl)) return false;
+    int ksh = bas_kl / nbas;
+    int lsh = bas_kl - ksh * nbas;
+    float d_cutoff = kl_cutoff - q_kl;
+    dmax = fmaxf(fmaxf(fmaxf(dm_cond[bas_ij], dm_cond[bas_kl]),
+                       fmaxf(dm_cond[ish*nbas+ksh], dm_cond[jsh*nbas+ksh])),
+                 fmaxf(dm_cond[ish*nbas+lsh], dm_cond[jsh*nbas+lsh]));
+    return dmax > d_cutoff;
+}
+
 // Work item of a launch: pair_ij index and the kl range of its chunk.
 struct WorkItem { int pij, kl0, kl1; };
 
@@ -267,11 +310,45 @@ __device__ __forceinline__ WorkItem work_item(int npairs_kl, int kl_chunk)
 // Primitive-pair record, see jk_pair_data(): (a, 1/a, Px, Py, Pz, K).
 #define PREC 6
 
-// One quartet per lane: returns the Cartesian block (times the symmetry and
-// normalisation prefactors) in gout[((i*nfj+j)*nfk+k)*nfl+l].
+// Rys roots in FP32: Chebyshev series in float on the float copy of the
+// table; the large-x asymptotic branch is evaluated in FP64.
+template <int N>
+__device__ __forceinline__ void rys_roots(const JKRysTable &tab, float x, float *r, float *w)
+{
+    if (x >= (float)tab.xmax[N-1]) {
+        double xd = x;
+        const double *lr = tab.coef + tab.large_r + N * (N - 1) / 2;
+        const double *lw = tab.coef + tab.large_w + N * (N - 1) / 2;
+        double ix = 1. / xd;
+        double sx = sqrt(ix);
+#pragma unroll
+        for (int i = 0; i < N; ++i) { r[i] = (float)(lr[i] * ix); w[i] = (float)(lw[i] * sx); }
+        return;
+    }
+    const int D1 = tab.degree + 1;
+    int k = (int)x;
+    float u = 2.f * (x - k) - 1.f;
+    float u2 = 2.f * u;
+    const float *c = tab.coef32 + tab.off[N-1] + (int64_t)k * (2 * N) * D1;
+#pragma unroll
+    for (int f = 0; f < 2 * N; ++f) {
+        const float *cf = c + f * D1;
+        float b1 = 0.f, b2 = 0.f;
+        for (int j = D1 - 1; j > 0; --j) {
+            float t = fmaf(u2, b1, cf[j] - b2);
+            b2 = b1; b1 = t;
+        }
+        float v = fmaf(u, b1, cf[0] - b2);
+        if (f & 1) w[f >> 1] = v; else r[f >> 1] = v;
+    }
+}
+
+// One quartet per lane in arithmetic type T (double or float): returns the
+// Cartesian block (times the symmetry and normalisation prefactors) in
+// gout[((i*nfj+j)*nfk+k)*nfl+l].
 // pij: records of the (uniform) ij pair; pkl: records of this lane's kl pair.
-template <int LI, int LJ, int LK, int LL>
-__device__ __forceinline__ void eri_lane(double *gout, const double *pij, int nij,
+template <int LI, int LJ, int LK, int LL, typename T>
+__device__ __forceinline__ void eri_lane(T *gout, const double *pij, int nij,
                                          const double *pkl, int nkl,
                                          const double *ri, const double *rij,
                                          const double *rk, const double *rkl,
@@ -285,42 +362,46 @@ __device__ __forceinline__ void eri_lane(double *gout, const double *pij, int ni
     constexpr CartPow<LJ> cj{};
     constexpr CartPow<LK> ck{};
     constexpr CartPow<LL> cl{};
+    const T rij_t[3] = {(T)rij[0], (T)rij[1], (T)rij[2]};
+    const T rkl_t[3] = {(T)rkl[0], (T)rkl[1], (T)rkl[2]};
 #pragma unroll
-    for (int n = 0; n < NF; ++n) gout[n] = 0.;
+    for (int n = 0; n < NF; ++n) gout[n] = 0;
     for (int kp = 0; kp < nkl; ++kp) {
         const double *q = pkl + kp * PREC;
-        const double akl = q[0], rakl = q[1];
-        const double Q[3] = {q[2], q[3], q[4]};
-        const double Kkl = q[5] * fac_sym;
-        double QC[3];
+        const T akl = (T)q[0], rakl = (T)q[1];
+        const T Kkl = (T)(q[5] * fac_sym);
+        T QC[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) QC[d] = Q[d] - rk[d];
+        for (int d = 0; d < 3; ++d) QC[d] = (T)(q[2+d] - rk[d]);
         for (int ip = 0; ip < nij; ++ip) {
             const double *p = pij + ip * PREC;
-            const double aij = p[0], raij = p[1];
-            double PA[3], PQ[3];
+            const T aij = (T)p[0], raij = (T)p[1];
+            T PA[3], PQ[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) { PA[d] = p[2+d] - ri[d]; PQ[d] = p[2+d] - Q[d]; }
-            double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
-            double r_aa = 1. / (aij + akl);
-            double theta = aij * akl * r_aa;
-            double fac = p[5] * Kkl * sqrt(r_aa);
-            double rt[N], wt[N];
+            for (int d = 0; d < 3; ++d) {
+                PA[d] = (T)(p[2+d] - ri[d]);
+                PQ[d] = (T)(p[2+d] - q[2+d]);
+            }
+            T rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
+            T r_aa = (T)1 / (aij + akl);
+            T theta = aij * akl * r_aa;
+            T fac = (T)p[5] * Kkl * sqrt(r_aa);
+            T rt[N], wt[N];
             rys_roots<N>(rys, theta * rr, rt, wt);
 #pragma unroll
             for (int ir = 0; ir < N; ++ir) {
-                double rt_aa = rt[ir] * r_aa;
-                double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
-                double b10 = .5 * raij * (1. - rt_aij);
-                double b01 = .5 * rakl * (1. - rt_akl);
-                double b00 = .5 * rt_aa;
-                double gx[GS], gy[GS], gz[GS];
-                build_g<LI, LJ, LK, LL>(gx, 1., PA[0] - rt_aij * PQ[0], QC[0] + rt_akl * PQ[0],
-                                        b10, b01, b00, rij[0], rkl[0]);
-                build_g<LI, LJ, LK, LL>(gy, 1., PA[1] - rt_aij * PQ[1], QC[1] + rt_akl * PQ[1],
-                                        b10, b01, b00, rij[1], rkl[1]);
+                T rt_aa = rt[ir] * r_aa;
+                T rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
+                T b10 = (T).5 * raij * ((T)1 - rt_aij);
+                T b01 = (T).5 * rakl * ((T)1 - rt_akl);
+                T b00 = (T).5 * rt_aa;
+                T gx[GS], gy[GS], gz[GS];
+                build_g<LI, LJ, LK, LL>(gx, (T)1, PA[0] - rt_aij * PQ[0], QC[0] + rt_akl * PQ[0],
+                                        b10, b01, b00, rij_t[0], rkl_t[0]);
+                build_g<LI, LJ, LK, LL>(gy, (T)1, PA[1] - rt_aij * PQ[1], QC[1] + rt_akl * PQ[1],
+                                        b10, b01, b00, rij_t[1], rkl_t[1]);
                 build_g<LI, LJ, LK, LL>(gz, wt[ir] * fac, PA[2] - rt_aij * PQ[2],
-                                        QC[2] + rt_akl * PQ[2], b10, b01, b00, rij[2], rkl[2]);
+                                        QC[2] + rt_akl * PQ[2], b10, b01, b00, rij_t[2], rkl_t[2]);
 #pragma unroll
                 for (int i = 0; i < NFI; ++i)
 #pragma unroll
@@ -341,26 +422,182 @@ __device__ __forceinline__ void eri_lane(double *gout, const double *pij, int ni
     }
 }
 
+// Evaluates and digests one batch of up to 64 queued quartets (one per lane)
+// in arithmetic type T; digestion and accumulation are FP64.
+template <int LI, int LJ, int LK, int LL, typename T>
+__device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair &sp,
+                                           uint32_t bas_ij, int pij, const uint32_t *queue,
+                                           int nb, int lane, uint64_t lt_mask, double *kacc)
+{
+    constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
+    constexpr int NF = NFI * NFJ * NFK * NFL;
+    const int nbas = a.nbas, nao = a.nao;
+    const size_t nao2 = (size_t)nao * nao;
+    const bool active = lane < nb;
+    const int pkl = queue[active ? lane : 0];
+    const uint32_t bas_kl = a.pairs_kl[pkl];
+    const int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
+    double fac_sym = 1.;
+    if (sp.ish == sp.jsh) fac_sym *= .5;
+    if (ksh == lsh) fac_sym *= .5;
+    if (bas_ij == bas_kl) fac_sym *= .5;
+    if (!active) fac_sym = 0.;
+    const double *rk = a.env + a.bas[ksh * 8 + 7], *rl = a.env + a.bas[lsh * 8 + 7];
+    const double rk3[3] = {rk[0], rk[1], rk[2]};
+    const double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
+    const int nij = a.iprim * a.jprim, nkl = a.kprim * a.lprim;
+    T gout[NF];
+    eri_lane<LI, LJ, LK, LL, T>(gout, a.pdata_ij + (size_t)pij * nij * PREC, nij,
+                                a.pdata_kl + (size_t)pkl * nkl * PREC, nkl,
+                                sp.ri, sp.rij, rk3, rkl, a.rys, fac_sym);
+    const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
+    const int i0 = sp.i0, j0 = sp.j0;
+    // runs of equal k shell are contiguous in the queue (tile order):
+    // segment = run id, from the run heads
+    const int key = active ? ksh : -1;
+    const int key_prev = __shfl_up(key, 1, WAVE);
+    const bool head = lane == 0 || key_prev != key;
+    const uint64_t heads = __ballot(head);
+    const int run = __popcll(heads & (lt_mask | (1ull << lane)));
+    const int run_next = __shfl_down(run, 1, WAVE);
+    const bool tail = active && (lane == nb - 1 || run_next != run);
+    for (int id = 0; id < a.n_dm; ++id) {
+        const double *D = a.dm + id * nao2;
+        if (a.vk) {
+            double *K = a.vk + id * nao2;
+            // K_ik += (ij|kl) D_jl ; K_il += (ij|kl) D_jk
+#pragma unroll
+            for (int i = 0; i < NFI; ++i) {
+                double kil[NFL], kik[NFK];
+#pragma unroll
+                for (int l = 0; l < NFL; ++l) kil[l] = 0.;
+#pragma unroll
+                for (int k = 0; k < NFK; ++k) kik[k] = 0.;
+                if (active) {
+#pragma unroll
+                    for (int j = 0; j < NFJ; ++j)
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) {
+                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
+                        kil[l] = fma(g, D[(j0+j)*nao + k0+k], kil[l]);
+                        kik[k] = fma(g, D[(j0+j)*nao + l0+l], kik[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < NFK; ++k) {
+                    double v = seg_sum(kik[k], run, lane);
+                    if (tail) atomic_add(K + (i0+i)*nao + k0+k, v);
+                }
+                if (active) {
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) {
+                        if (kacc) atomicAdd(kacc + i * a.l_nao + l0 + l - a.l_ao0, kil[l]);
+                        else atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
+                    }
+                }
+            }
+            // K_jk += (ij|kl) D_il ; K_jl += (ij|kl) D_ik
+#pragma unroll
+            for (int j = 0; j < NFJ; ++j) {
+                double kjl[NFL], kjk[NFK];
+#pragma unroll
+                for (int l = 0; l < NFL; ++l) kjl[l] = 0.;
+#pragma unroll
+                for (int k = 0; k < NFK; ++k) kjk[k] = 0.;
+                if (active) {
+#pragma unroll
+                    for (int i = 0; i < NFI; ++i)
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) {
+                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
+                        kjl[l] = fma(g, D[(i0+i)*nao + k0+k], kjl[l]);
+                        kjk[k] = fma(g, D[(i0+i)*nao + l0+l], kjk[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < NFK; ++k) {
+                    double v = seg_sum(kjk[k], run, lane);
+                    if (tail) atomic_add(K + (j0+j)*nao + k0+k, v);
+                }
+                if (active) {
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) {
+                        if (kacc) atomicAdd(kacc + (NFI + j) * a.l_nao + l0 + l - a.l_ao0, kjl[l]);
+                        else atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
+                    }
+                }
+            }
+        }
+        if (a.vj) {
+            double *J = a.vj + id * nao2;
+            // J_kl += (ij|kl) D_ij   (per lane)
+            if (active) {
+#pragma unroll
+                for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                for (int l = 0; l < NFL; ++l) {
+                    double s = 0.;
+#pragma unroll
+                    for (int i = 0; i < NFI; ++i)
+#pragma unroll
+                    for (int j = 0; j < NFJ; ++j)
+                        s = fma((double)gout[((i*NFJ + j)*NFK + k)*NFL + l],
+                                D[(i0+i)*nao + j0+j], s);
+                    atomic_add(J + (k0+k)*nao + l0+l, s);
+                }
+            }
+            // J_ij += (ij|kl) D_kl   (common ij: reduce over the wave)
+            double vij[NFI*NFJ];
+#pragma unroll
+            for (int i = 0; i < NFI; ++i)
+#pragma unroll
+            for (int j = 0; j < NFJ; ++j) {
+                double s = 0.;
+                if (active) {
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l)
+                        s = fma((double)gout[((i*NFJ + j)*NFK + k)*NFL + l],
+                                D[(k0+k)*nao + l0+l], s);
+                }
+                vij[i*NFJ + j] = wave_sum(s);
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int n = 0; n < NFI*NFJ; ++n)
+                    atomic_add(J + (i0 + n / NFJ)*nao + j0 + n % NFJ, vij[n]);
+            }
+        }
+    }
+}
+
+// Per-lane kernel.  LANE_WAVES wavefronts share one work item (and its LDS K
+// rows); each wavefront scans every LANE_WAVES-th batch of 64 kl candidates
+// and keeps two queues: quartets whose bound q_ij + q_kl + max(dm_cond)
+// reaches fp32_cut run in FP64, the others in FP32.
 template <int LI, int LJ, int LK, int LL>
 __global__ __launch_bounds__(LANE_WAVES * WAVE)
 void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
 {
-    constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
-    constexpr int NF = NFI * NFJ * NFK * NFL;
-    // LANE_WAVES wavefronts share one work item (and its LDS K rows); each
-    // wavefront scans every LANE_WAVES-th batch of 64 kl candidates
-    __shared__ uint32_t queue_all[LANE_WAVES][2 * WAVE];
+    constexpr int NFI = ncart(LI), NFJ = ncart(LJ);
+    __shared__ uint32_t queue_all[LANE_WAVES][2][2 * WAVE];
     const int lane = threadIdx.x & (WAVE - 1);
     const int wid = threadIdx.x / WAVE;
-    uint32_t *queue = queue_all[wid];
+    uint32_t *queue64 = queue_all[wid][0];
+    uint32_t *queue32 = queue_all[wid][1];
     const int nbas = a.nbas, nao = a.nao;
     WorkItem wi = work_item(a.npairs_kl, kl_chunk);
     const uint32_t bas_ij = a.pairs_ij[wi.pij];
-    const float kl_cutoff = a.cutoff - a.q_ij[wi.pij];
+    const float q_ij = a.q_ij[wi.pij];
+    const float kl_cutoff = a.cutoff - q_ij;
     if (a.q_kl[wi.kl0] + a.dm_penalty + Q_COND_MARGIN < kl_cutoff) return;
     ShellPair sp;
     load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
-    const size_t nao2 = (size_t)nao * nao;
     // K_il / K_jl rows of this work item accumulated in LDS (one density)
     extern __shared__ double kacc_dyn[];
     double *kacc = use_lds ? kacc_dyn : nullptr;
@@ -369,11 +606,186 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
         for (int n = threadIdx.x; n < nacc; n += LANE_WAVES * WAVE) kacc[n] = 0.;
         __syncthreads();
     }
+    int qn64 = 0, qn32 = 0;
+    unsigned long long nsurv = 0, nsurv32 = 0;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    bool stop = false;
+    for (int base = wi.kl0 + wid * WAVE; base < wi.kl1 || qn64 > 0 || qn32 > 0;
+         base += LANE_WAVES * WAVE) {
+        if (base < wi.kl1 && !stop) {
+            int pkl = base + lane;
+            bool keep = false, fail_margin = false, lowp = false;
+            if (pkl < wi.kl1) {
+                float q_kl = a.q_kl[pkl];
+                fail_margin = q_kl + a.dm_penalty + Q_COND_MARGIN < kl_cutoff;
+                float dmax;
+                keep = survives_dmax(q_kl, kl_cutoff, a.dm_penalty, bas_ij, a.pairs_kl[pkl],
+                                     sp.ish, sp.jsh, nbas, a.dm_cond, dmax);
+                lowp = keep && (q_ij + q_kl + dmax < a.fp32_cut);
+            }
+            const uint64_t m64 = __ballot(keep && !lowp);
+            const uint64_t m32 = __ballot(lowp);
+            if (keep && !lowp) queue64[qn64 + __popcll(m64 & lt_mask)] = pkl;
+            if (lowp) queue32[qn32 + __popcll(m32 & lt_mask)] = pkl;
+            qn64 += __popcll(m64);
+            qn32 += __popcll(m32);
+            nsurv += __popcll(m64) + __popcll(m32);
+            nsurv32 += __popcll(m32);
+            stop = __ballot(fail_margin) != 0;
+            wave_sync();
+        }
+        const bool flush = (base + LANE_WAVES * WAVE >= wi.kl1) || stop;
+        if (qn64 >= WAVE || (flush && qn64 > 0)) {
+            const int nb = min(qn64, WAVE);
+            lane_batch<LI, LJ, LK, LL, double>(a, sp, bas_ij, wi.pij, queue64, nb, lane,
+                                               lt_mask, kacc);
+            wave_sync();
+            uint32_t rest = (nb + lane < qn64) ? queue64[nb + lane] : 0u;
+            wave_sync();
+            if (nb + lane < qn64) queue64[lane] = rest;
+            qn64 -= nb;
+            wave_sync();
+        }
+        if (qn32 >= WAVE || (flush && qn32 > 0)) {
+            const int nb = min(qn32, WAVE);
+            lane_batch<LI, LJ, LK, LL, float>(a, sp, bas_ij, wi.pij, queue32, nb, lane,
+                                              lt_mask, kacc);
+            wave_sync();
+            uint32_t rest = (nb + lane < qn32) ? queue32[nb + lane] : 0u;
+            wave_sync();
+            if (nb + lane < qn32) queue32[lane] = rest;
+            qn32 -= nb;
+            wave_sync();
+        }
+        if (flush && qn64 == 0 && qn32 == 0) break;
+    }
+    if (kacc) {
+        __syncthreads();
+        for (int n = threadIdx.x; n < nacc; n += LANE_WAVES * WAVE) {
+            double v = kacc[n];
+            if (v != 0.) {
+                int r = n / a.l_nao, c = n - r * a.l_nao;
+                int row = r < NFI ? sp.i0 + r : sp.j0 + r - NFI;
+                atomic_add(a.vk + (size_t)row * nao + a.l_ao0 + c, v);
+            }
+        }
+    }
+    if (lane == 0) {
+        if (a.n_quartets && nsurv) atomicAdd(a.n_quartets, nsurv);
+        if (a.n_quartets_fp32 && nsurv32) atomicAdd(a.n_quartets_fp32, nsurv32);
+    }
+}
+
+// ---------------------------------------------------------------- team
+// Larger classes: teams of TS lanes inside one wavefront evaluate one quartet
+// each; every wavefront of the block scans and evaluates its own kl
+// candidates, so only LDS ordering within a wavefront is needed.
+// Per primitive quartet: lanes r < N evaluate one Rys root each, lanes
+// t < 3N build the 2D table of (root t/3, direction t%3) into LDS, then each
+// lane owns one or two (i,j) rows of the block and accumulates all (k,l).
+template <int LI, int LJ, int LK, int LL>
+struct TeamShape {
+    static constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
+    static constexpr int NROWS = NFI * NFJ;
+    static constexpr int NFKL = NFK * NFL;
+    static constexpr int N = (LI + LJ + LK + LL) / 2 + 1;
+    static constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
+    static constexpr int RPL = NROWS > WAVE ? 2 : 1;                 // rows per lane
+    static constexpr int TNEED0 = (NROWS + RPL - 1) / RPL;
+    static constexpr int TNEED = TNEED0 > 3 * N ? TNEED0 : 3 * N;
+    static constexpr int NQW = TNEED <= 16 ? 4 : (TNEED <= 32 ? 2 : 1);   // teams per wave
+    static constexpr int TS = WAVE / NQW;
+    // per-team LDS (doubles): roots, 2D tables, J_kl, K_ik, K_jk, K_il, K_jl
+    static constexpr int O_RW = 0;
+    static constexpr int O_G = 2 * N;
+    static constexpr int O_JKL = O_G + N * 3 * GS;
+    static constexpr int O_KIK = O_JKL + NFKL;
+    static constexpr int O_KJK = O_KIK + NFI * NFK;
+    static constexpr int O_KIL = O_KJK + NFJ * NFK;
+    static constexpr int O_KJL = O_KIL + NFI * NFL;
+    static constexpr int PER_TEAM = O_KJL + NFJ * NFL;
+    static constexpr int PER_WAVE = NQW * PER_TEAM;
+    // waves per block within ~47 KB of static LDS
+    static constexpr int WB0 = 6000 / PER_WAVE;
+    static constexpr int WB = WB0 >= 4 ? 4 : (WB0 >= 2 ? 2 : 1);
+    static constexpr int TB = WB * WAVE;
+    static_assert(WB * PER_WAVE * 8 + WB * 2 * WAVE * 4 <= 48 * 1024,
+                  "team kernel static LDS above 48 KB");
+};
+
+template <int LI, int LJ, int LK, int LL>
+__global__ __launch_bounds__((TeamShape<LI, LJ, LK, LL>::TB))
+void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
+{
+    using S = TeamShape<LI, LJ, LK, LL>;
+    constexpr int NFI = S::NFI, NFJ = S::NFJ, NFK = S::NFK, NFL = S::NFL;
+    constexpr int NROWS = S::NROWS, NFKL = S::NFKL, N = S::N, GS = S::GS;
+    constexpr int RPL = S::RPL, NQW = S::NQW, TS = S::TS, WB = S::WB;
+    constexpr CartPow<LI> ci{};
+    constexpr CartPow<LJ> cj{};
+    constexpr CartPow<LK> ck{};
+    constexpr CartPow<LL> cl{};
+    constexpr int SJ = LL+1, SK = (LK+1)*(LL+1), SI = (LJ+1)*(LK+1)*(LL+1);
+
+    __shared__ uint32_t queue_all[WB][2 * WAVE];
+    __shared__ double lds[WB * S::PER_WAVE];
+    extern __shared__ double kacc_dyn[];
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = threadIdx.x / WAVE;
+    const int team = lane / TS;
+    const int tl = lane - team * TS;
+    uint32_t *queue = queue_all[wid];
+    double *my = lds + wid * S::PER_WAVE + team * S::PER_TEAM;
+    double *rw = my + S::O_RW, *gbuf = my + S::O_G, *jkl = my + S::O_JKL;
+    double *kik = my + S::O_KIK, *kjk = my + S::O_KJK;
+    double *kil = my + S::O_KIL, *kjl = my + S::O_KJL;
+
+    const int nbas = a.nbas, nao = a.nao;
+    WorkItem wi = work_item(a.npairs_kl, kl_chunk);
+    const uint32_t bas_ij = a.pairs_ij[wi.pij];
+    const float kl_cutoff = a.cutoff - a.q_ij[wi.pij];
+    if (a.q_kl[wi.kl0] + a.dm_penalty + Q_COND_MARGIN < kl_cutoff) return;
+    ShellPair sp;
+    load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
+    const size_t nao2 = (size_t)nao * nao;
+    const int i0 = sp.i0, j0 = sp.j0;
+    double *kacc = use_lds ? kacc_dyn : nullptr;      // K_il / K_jl rows (one density)
+    const int nacc = (NFI + NFJ) * a.l_nao;
+    if (kacc) {
+        for (int n = threadIdx.x; n < nacc; n += WB * WAVE) kacc[n] = 0.;
+    }
+    __syncthreads();
+
+    // rows owned by this lane
+    int rows[RPL];
+    int rpow[RPL][6];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+        int row = tl + q * TS;
+        rows[q] = row < NROWS ? row : -1;
+        int ir = row < NROWS ? row / NFJ : 0, jr = row < NROWS ? row % NFJ : 0;
+        int px = 0, py = 0, pz = 0, qx = 0, qy = 0, qz = 0;
+#pragma unroll
+        for (int i = 0; i < NFI; ++i)
+            if (i == ir) { px = ci.x[i]; py = ci.y[i]; pz = ci.z[i]; }
+#pragma unroll
+        for (int j = 0; j < NFJ; ++j)
+            if (j == jr) { qx = cj.x[j]; qy = cj.y[j]; qz = cj.z[j]; }
+        rpow[q][0] = px; rpow[q][1] = py; rpow[q][2] = pz;
+        rpow[q][3] = qx; rpow[q][4] = qy; rpow[q][5] = qz;
+    }
+    double vjij[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) vjij[q] = 0.;
+    int cur_k = -1;            // k shell of the K_ik / K_jk run held in LDS
+    for (int n = tl; n < NFI * NFK + NFJ * NFK; n += TS) kik[n] = 0.;   // kik, kjk adjacent
+
     int qn = 0;
     unsigned long long nsurv = 0;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     bool stop = false;
-    for (int base = wi.kl0 + wid * WAVE; base < wi.kl1 || qn > 0; base += LANE_WAVES * WAVE) {
+    for (int base = wi.kl0 + wid * WAVE; base < wi.kl1 || qn > 0; base += WB * WAVE) {
         if (base < wi.kl1 && !stop) {
             int pkl = base + lane;
             bool keep = false, fail_margin = false;
@@ -390,296 +802,26 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
             stop = __ballot(fail_margin) != 0;
             wave_sync();
         }
-        bool flush = (base + LANE_WAVES * WAVE >= wi.kl1) || stop;
-        if (qn == 0) {
-            // nothing queued: the queue holds no valid entry to read
-            if (flush) break;
-            continue;
-        }
-        if (qn < WAVE && !flush) continue;
-        int nb = min(qn, WAVE);
-        // ---- evaluate up to 64 queued quartets, one per lane
-        double vij[NFI*NFJ];
-        bool active = lane < nb;
-        const int pkl = queue[active ? lane : 0];
-        uint32_t bas_kl = a.pairs_kl[pkl];
-        int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
-        double fac_sym = 1.;
-        if (sp.ish == sp.jsh) fac_sym *= .5;
-        if (ksh == lsh) fac_sym *= .5;
-        if (bas_ij == bas_kl) fac_sym *= .5;
-        if (!active) fac_sym = 0.;
-        const double *rk = a.env + a.bas[ksh * 8 + 7], *rl = a.env + a.bas[lsh * 8 + 7];
-        const double rk3[3] = {rk[0], rk[1], rk[2]};
-        const double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
-        const int nij = a.iprim * a.jprim, nkl = a.kprim * a.lprim;
-        double gout[NF];
-        eri_lane<LI, LJ, LK, LL>(gout, a.pdata_ij + (size_t)wi.pij * nij * PREC, nij,
-                                 a.pdata_kl + (size_t)pkl * nkl * PREC, nkl,
-                                 sp.ri, sp.rij, rk3, rkl, a.rys, fac_sym);
-        const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
-        const int i0 = sp.i0, j0 = sp.j0;
-        // runs of equal k shell are contiguous in the queue (tile order):
-        // segment = run id, from the run heads
-        const int key = active ? ksh : -1;
-        const int key_prev = __shfl_up(key, 1, WAVE);
-        const bool head = lane == 0 || key_prev != key;
-        const uint64_t heads = __ballot(head);
-        const int run = __popcll(heads & (lt_mask | (1ull << lane)));
-        const int run_next = __shfl_down(run, 1, WAVE);
-        const bool tail = active && (lane == nb - 1 || run_next != run);
-        for (int id = 0; id < a.n_dm; ++id) {
-            const double *D = a.dm + id * nao2;
-            if (a.vk) {
-                double *K = a.vk + id * nao2;
-                // K_ik += (ij|kl) D_jl ; K_il += (ij|kl) D_jk
-#pragma unroll
-                for (int i = 0; i < NFI; ++i) {
-                    double kil[NFL], kik[NFK];
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l) kil[l] = 0.;
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k) kik[k] = 0.;
-                    if (active) {
-#pragma unroll
-                        for (int j = 0; j < NFJ; ++j)
-#pragma unroll
-                        for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                        for (int l = 0; l < NFL; ++l) {
-                            double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
-                            kil[l] = fma(g, D[(j0+j)*nao + k0+k], kil[l]);
-                            kik[k] = fma(g, D[(j0+j)*nao + l0+l], kik[k]);
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k) {
-                        double v = seg_sum(kik[k], run, lane);
-                        if (tail) atomic_add(K + (i0+i)*nao + k0+k, v);
-                    }
-                    if (active) {
-#pragma unroll
-                        for (int l = 0; l < NFL; ++l) {
-                            if (kacc) atomicAdd(kacc + i * a.l_nao + l0 + l - a.l_ao0, kil[l]);
-                            else atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
-                        }
-                    }
-                }
-                // K_jk += (ij|kl) D_il ; K_jl += (ij|kl) D_ik
-#pragma unroll
-                for (int j = 0; j < NFJ; ++j) {
-                    double kjl[NFL], kjk[NFK];
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l) kjl[l] = 0.;
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k) kjk[k] = 0.;
-                    if (active) {
-#pragma unroll
-                        for (int i = 0; i < NFI; ++i)
-#pragma unroll
-                        for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                        for (int l = 0; l < NFL; ++l) {
-                            double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
-                            kjl[l] = fma(g, D[(i0+i)*nao + k0+k], kjl[l]);
-                            kjk[k] = fma(g, D[(i0+i)*nao + l0+l], kjk[k]);
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k) {
-                        double v = seg_sum(kjk[k], run, lane);
-                        if (tail) atomic_add(K + (j0+j)*nao + k0+k, v);
-                    }
-                    if (active) {
-#pragma unroll
-                        for (int l = 0; l < NFL; ++l) {
-                            if (kacc) atomicAdd(kacc + (NFI + j) * a.l_nao + l0 + l - a.l_ao0, kjl[l]);
-                            else atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
-                        }
-                    }
-                }
-            }
-            if (a.vj) {
-                double *J = a.vj + id * nao2;
-                // J_kl += (ij|kl) D_ij   (per lane)
-                if (active) {
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l) {
-                        double s = 0.;
-#pragma unroll
-                        for (int i = 0; i < NFI; ++i)
-#pragma unroll
-                        for (int j = 0; j < NFJ; ++j)
-                            s = fma(gout[((i*NFJ + j)*NFK + k)*NFL + l],
-                                    D[(i0+i)*nao + j0+j], s);
-                        atomic_add(J + (k0+k)*nao + l0+l, s);
-                    }
-                }
-                // J_ij += (ij|kl) D_kl   (common ij: reduce over the wave)
-#pragma unroll
-                for (int i = 0; i < NFI; ++i)
-#pragma unroll
-                for (int j = 0; j < NFJ; ++j) {
-                    double s = 0.;
-                    if (active) {
-#pragma unroll
-                        for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                        for (int l = 0; l < NFL; ++l)
-                            s = fma(gout[((i*NFJ + j)*NFK + k)*NFL + l],
-                                    D[(k0+k)*nao + l0+l], s);
-                    }
-                    vij[i*NFJ + j] = wave_sum(s);
-                }
-                if (lane == 0) {
-#pragma unroll
-                    for (int n = 0; n < NFI*NFJ; ++n)
-                        atomic_add(J + (i0 + n / NFJ)*nao + j0 + n % NFJ, vij[n]);
-                }
-            }
-        }
-        // shift the remainder of the queue to the front
-        wave_sync();
-        uint32_t rest = (nb + lane < qn) ? queue[nb + lane] : 0u;
-        wave_sync();
-        if (nb + lane < qn) queue[lane] = rest;
-        qn -= nb;
-        wave_sync();
-        if (flush && qn == 0) break;
-    }
-    if (kacc) {
-        __syncthreads();
-        for (int n = threadIdx.x; n < nacc; n += LANE_WAVES * WAVE) {
-            double v = kacc[n];
-            if (v != 0.) {
-                int r = n / a.l_nao, c = n - r * a.l_nao;
-                int row = r < NFI ? sp.i0 + r : sp.j0 + r - NFI;
-                atomic_add(a.vk + (size_t)row * nao + a.l_ao0 + c, v);
-            }
-        }
-    }
-    if (a.n_quartets && lane == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
-}
-
-// ---------------------------------------------------------------- team
-// Team of TS lanes per quartet, NQ quartets per block of TB threads.
-template <int LI, int LJ, int LK, int LL>
-struct TeamShape {
-    static constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
-    static constexpr int NROWS = NFI * NFJ;
-    static constexpr int NFKL = NFK * NFL;
-    static constexpr int N = (LI + LJ + LK + LL) / 2 + 1;
-    static constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
-    static constexpr int TS0 = NROWS > 3 * N ? NROWS : 3 * N;
-    static constexpr int TB = TS0 <= 64 ? 64 : (TS0 <= 128 ? 128 : 256);
-    static constexpr int G_DBL = 2 * N + N * 3 * GS;            // roots + 2D integrals
-    static constexpr int S_ROWS0 = 4096 / NFKL;                 // rows staged per pass
-    static constexpr int S_ROWS = S_ROWS0 < NROWS ? (S_ROWS0 > 0 ? S_ROWS0 : 1) : NROWS;
-    static constexpr int S_DBL = S_ROWS * NFKL;
-    static constexpr int PER_Q = G_DBL > S_DBL ? G_DBL : S_DBL;
-    static constexpr int NQ0 = TB / TS0;
-    static constexpr int NQ1 = (6144 / PER_Q) > 0 ? (6144 / PER_Q) : 1;   // 48 KB of LDS
-    static constexpr int NQ = NQ0 < NQ1 ? NQ0 : NQ1;
-    static constexpr int TS = TB / NQ;
-};
-
-template <int LI, int LJ, int LK, int LL>
-__global__ __launch_bounds__((TeamShape<LI, LJ, LK, LL>::TB))
-void jk_team_kernel(JKBuildArgs a, int kl_chunk)
-{
-    using S = TeamShape<LI, LJ, LK, LL>;
-    constexpr int NFI = S::NFI, NFJ = S::NFJ, NFK = S::NFK, NFL = S::NFL;
-    constexpr int NROWS = S::NROWS, NFKL = S::NFKL, N = S::N, GS = S::GS;
-    constexpr int TB = S::TB, NQ = S::NQ, TS = S::TS;
-    constexpr int NW = TB / WAVE;
-    constexpr CartPow<LI> ci{};
-    constexpr CartPow<LJ> cj{};
-    constexpr CartPow<LK> ck{};
-    constexpr CartPow<LL> cl{};
-    constexpr int SJ = LL+1, SK = (LK+1)*(LL+1), SI = (LJ+1)*(LK+1)*(LL+1);
-
-    __shared__ uint32_t queue[2 * TB];
-    __shared__ int wave_cnt[NW + 1];
-    __shared__ int stop_flag;
-    __shared__ double lds[NQ * S::PER_Q];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & (WAVE - 1);
-    const int wid = tid / WAVE;
-    const int team = tid / TS;
-    const int tl = tid - team * TS;
-    const int nbas = a.nbas, nao = a.nao;
-    const size_t nao2 = (size_t)nao * nao;
-    WorkItem wi = work_item(a.npairs_kl, kl_chunk);
-    const uint32_t bas_ij = a.pairs_ij[wi.pij];
-    const float kl_cutoff = a.cutoff - a.q_ij[wi.pij];
-    if (a.q_kl[wi.kl0] + a.dm_penalty + Q_COND_MARGIN < kl_cutoff) return;
-    ShellPair sp;
-    load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    double *my = lds + (team < NQ ? team : 0) * S::PER_Q;
-    double *rw = my;                  // [2N]
-    double *gbuf = my + 2 * N;        // [N][3][GS]
-    const int row = tl;               // (i, j) row owned by this lane
-    const bool has_row = team < NQ && row < NROWS;
-    const int irow = has_row ? row / NFJ : 0, jrow = has_row ? row % NFJ : 0;
-    int qn = 0;
-    unsigned long long nsurv = 0;
-    bool stop = false;
-    for (int base = wi.kl0; ; base += TB) {
-        if (base < wi.kl1 && !stop) {
-            int pkl = base + tid;
-            bool keep = false, fail_margin = false;
-            uint32_t bkl = 0;
-            if (pkl < wi.kl1) {
-                float q_kl = a.q_kl[pkl];
-                bkl = a.pairs_kl[pkl];
-                fail_margin = q_kl + a.dm_penalty + Q_COND_MARGIN < kl_cutoff;
-                keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
-                                sp.ish, sp.jsh, nbas, a.dm_cond);
-            }
-            uint64_t m = __ballot(keep);
-            uint64_t mf = __ballot(fail_margin);
-            if (tid == 0) stop_flag = 0;
-            __syncthreads();
-            if (lane == 0) {
-                wave_cnt[wid] = __popcll(m);
-                if (mf) stop_flag = 1;
-            }
-            __syncthreads();
-            int off = qn;
-            for (int w = 0; w < wid; ++w) off += wave_cnt[w];
-            if (keep) queue[off + __popcll(m & lt_mask)] = pkl;
-            int tot = 0;
-            for (int w = 0; w < NW; ++w) tot += wave_cnt[w];
-            qn += tot;
-            nsurv += tot;
-            stop = stop_flag != 0;
-            __syncthreads();
-        }
-        bool flush = base + TB >= wi.kl1 || stop;
-        int head = 0;
-        while (qn - head >= NQ || (flush && head < qn)) {
-            // team `team` evaluates queue[head + team]
-            const int qidx = head + team;
-            const bool active = team < NQ && qidx < qn;
-            const int pkl = queue[active ? qidx : head];
-            uint32_t bas_kl = a.pairs_kl[pkl];
-            int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
+        const bool flush = (base + WB * WAVE >= wi.kl1) || stop;
+        while (qn >= NQW || (flush && qn > 0)) {
+            const bool active = team < qn;
+            const int pkl = queue[active ? team : 0];
+            const uint32_t bas_kl = a.pairs_kl[pkl];
+            const int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
             double fac_sym = 1.;
             if (sp.ish == sp.jsh) fac_sym *= .5;
             if (ksh == lsh) fac_sym *= .5;
             if (bas_ij == bas_kl) fac_sym *= .5;
             const double *rk = a.env + a.bas[ksh * 8 + 7], *rl = a.env + a.bas[lsh * 8 + 7];
-            double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
+            const double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
             const int nij = a.iprim * a.jprim, nkl = a.kprim * a.lprim;
             const double *pdij = a.pdata_ij + (size_t)wi.pij * nij * PREC;
             const double *pdkl = a.pdata_kl + (size_t)pkl * nkl * PREC;
-            double acc[NFKL];
+            double acc[RPL][NFKL];
 #pragma unroll
-            for (int n = 0; n < NFKL; ++n) acc[n] = 0.;
+            for (int q = 0; q < RPL; ++q)
+#pragma unroll
+                for (int n = 0; n < NFKL; ++n) acc[q][n] = 0.;
             for (int kp = 0; kp < nkl; ++kp) {
                 const double *qr = pdkl + kp * PREC;
                 const double akl = qr[0], rakl = qr[1];
@@ -695,171 +837,239 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
                         PQ[d] = pr[2+d] - Q[d];
                         QC[d] = Q[d] - rk[d];
                     }
-                    double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
-                    double r_aa = 1. / (aij + akl);
-                    double theta = aij * akl * r_aa;
-                    double fac = pr[5] * Kkl * sqrt(r_aa);
-                    __syncthreads();
-                    if (active && tl == 0) {
-                        double rt[N], wt[N];
-                        rys_roots<N>(a.rys, theta * rr, rt, wt);
-#pragma unroll
-                        for (int r = 0; r < N; ++r) { rw[2*r] = rt[r]; rw[2*r+1] = wt[r]; }
+                    const double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
+                    const double r_aa = 1. / (aij + akl);
+                    const double theta = aij * akl * r_aa;
+                    const double fac = pr[5] * Kkl * sqrt(r_aa);
+                    if (active && tl < N) {
+                        double r, w;
+                        rys_root_one<N>(a.rys, theta * rr, tl, r, w);
+                        rw[2*tl] = r; rw[2*tl+1] = w;
                     }
-                    __syncthreads();
+                    wave_sync();
                     if (active && tl < 3 * N) {
-                        int r = tl / 3, d = tl - (tl / 3) * 3;
-                        double rt_aa = rw[2*r] * r_aa;
-                        double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
-                        double b10 = .5 * raij * (1. - rt_aij);
-                        double b01 = .5 * rakl * (1. - rt_akl);
-                        double b00 = .5 * rt_aa;
+                        const int r = tl / 3, d = tl - (tl / 3) * 3;
+                        const double rt_aa = rw[2*r] * r_aa;
+                        const double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
+                        const double b10 = .5 * raij * (1. - rt_aij);
+                        const double b01 = .5 * rakl * (1. - rt_akl);
+                        const double b00 = .5 * rt_aa;
+                        const double pa = d == 0 ? PA[0] : (d == 1 ? PA[1] : PA[2]);
+                        const double pq = d == 0 ? PQ[0] : (d == 1 ? PQ[1] : PQ[2]);
+                        const double qc = d == 0 ? QC[0] : (d == 1 ? QC[1] : QC[2]);
+                        const double rd = d == 0 ? sp.rij[0] : (d == 1 ? sp.rij[1] : sp.rij[2]);
+                        const double kd = d == 0 ? rkl[0] : (d == 1 ? rkl[1] : rkl[2]);
+                        const double g00 = d == 2 ? rw[2*r+1] * fac : 1.;
                         double g[GS];
-                        double pa = d == 0 ? PA[0] : (d == 1 ? PA[1] : PA[2]);
-                        double pq = d == 0 ? PQ[0] : (d == 1 ? PQ[1] : PQ[2]);
-                        double qc = d == 0 ? QC[0] : (d == 1 ? QC[1] : QC[2]);
-                        double rd = d == 0 ? sp.rij[0] : (d == 1 ? sp.rij[1] : sp.rij[2]);
-                        double kd = d == 0 ? rkl[0] : (d == 1 ? rkl[1] : rkl[2]);
-                        double g00 = d == 2 ? rw[2*r+1] * fac : 1.;
-                        build_g<LI, LJ, LK, LL>(g, g00, pa - rt_aij * pq,
-                                                qc + rt_akl * pq,
-                                                b10, b01, b00, rd, kd);
+                        build_g<LI, LJ, LK, LL, double>(g, g00, pa - rt_aij * pq,
+                                                        qc + rt_akl * pq, b10, b01, b00, rd, kd);
                         double *dst = gbuf + (r * 3 + d) * GS;
 #pragma unroll
                         for (int n = 0; n < GS; ++n) dst[n] = g[n];
                     }
-                    __syncthreads();
-                    if (active && has_row) {
-                        int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
-                        // Cartesian powers of this row's i and j (runtime lookup)
+                    wave_sync();
+                    if (active) {
 #pragma unroll
-                        for (int i = 0; i < NFI; ++i)
-                            if (i == irow) { ax = ci.x[i]; ay = ci.y[i]; az = ci.z[i]; }
+                        for (int q = 0; q < RPL; ++q) {
+                            if (rows[q] < 0) continue;
+                            const int ox = rpow[q][0] * SI + rpow[q][3] * SK;
+                            const int oy = rpow[q][1] * SI + rpow[q][4] * SK;
+                            const int oz = rpow[q][2] * SI + rpow[q][5] * SK;
+                            for (int r = 0; r < N; ++r) {
+                                const double *gx = gbuf + (r * 3 + 0) * GS + ox;
+                                const double *gy = gbuf + (r * 3 + 1) * GS + oy;
+                                const double *gz = gbuf + (r * 3 + 2) * GS + oz;
+                                double sx[SK], sy[SK], sz[SK];
 #pragma unroll
-                        for (int j = 0; j < NFJ; ++j)
-                            if (j == jrow) { bx = cj.x[j]; by = cj.y[j]; bz = cj.z[j]; }
-                        const int ox = ax * SI + bx * SK, oy = ay * SI + by * SK, oz = az * SI + bz * SK;
-                        for (int r = 0; r < N; ++r) {
-                            const double *gx = gbuf + (r * 3 + 0) * GS + ox;
-                            const double *gy = gbuf + (r * 3 + 1) * GS + oy;
-                            const double *gz = gbuf + (r * 3 + 2) * GS + oz;
-                            double sx[SK], sy[SK], sz[SK];
+                                for (int n = 0; n < SK; ++n) { sx[n] = gx[n]; sy[n] = gy[n]; sz[n] = gz[n]; }
 #pragma unroll
-                            for (int n = 0; n < SK; ++n) { sx[n] = gx[n]; sy[n] = gy[n]; sz[n] = gz[n]; }
+                                for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                                for (int l = 0; l < NFL; ++l) {
+                                    const int kx = ck.x[k]*SJ + cl.x[l];
+                                    const int ky = ck.y[k]*SJ + cl.y[l];
+                                    const int kz = ck.z[k]*SJ + cl.z[l];
+                                    acc[q][k*NFL + l] = fma(sx[kx] * sy[ky], sz[kz], acc[q][k*NFL + l]);
+                                }
+                            }
+                        }
+                    }
+                    wave_sync();
+                }
+            }
+            // ---- digestion (one density through LDS, else direct atomics)
+            const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
+            for (int id = 0; id < a.n_dm; ++id) {
+                const double *D = a.dm + id * nao2;
+                const bool lds_k = kacc != nullptr;      // n_dm == 1 when set
+                if (a.vk) {
+                    double *K = a.vk + id * nao2;
+                    // a new k shell ends the K_ik / K_jk run: flush it
+                    const int run_k = active ? ksh : cur_k;
+                    if (lds_k && run_k != cur_k && cur_k >= 0) {
+                        const int kc0 = a.ao_loc[cur_k];
+                        for (int n = tl; n < NFI * NFK + NFJ * NFK; n += TS) {
+                            double v = kik[n];
+                            if (v != 0.) {
+                                int row = n < NFI * NFK ? i0 + n / NFK : j0 + (n - NFI * NFK) / NFK;
+                                int col = kc0 + (n < NFI * NFK ? n % NFK : (n - NFI * NFK) % NFK);
+                                atomic_add(K + (size_t)row * nao + col, v);
+                            }
+                            kik[n] = 0.;
+                        }
+                        wave_sync();
+                    }
+                    if (lds_k) cur_k = run_k;
+                    if (!lds_k) {
+                        for (int n = tl; n < NFI*NFK + NFJ*NFK + NFI*NFL + NFJ*NFL; n += TS) kik[n] = 0.;
+                        wave_sync();
+                    }
+                    if (active) {
+#pragma unroll
+                        for (int q = 0; q < RPL; ++q) {
+                            if (rows[q] < 0) continue;
+                            const int ir = rows[q] / NFJ, jr = rows[q] % NFJ;
+                            // K_ik += sum_l D_jl ; K_jk += sum_l D_il
+#pragma unroll
+                            for (int k = 0; k < NFK; ++k) {
+                                double s1 = 0., s2 = 0.;
+#pragma unroll
+                                for (int l = 0; l < NFL; ++l) {
+                                    s1 = fma(acc[q][k*NFL + l], D[(j0+jr)*nao + l0+l], s1);
+                                    s2 = fma(acc[q][k*NFL + l], D[(i0+ir)*nao + l0+l], s2);
+                                }
+                                atomicAdd(kik + ir * NFK + k, s1);
+                                atomicAdd(kjk + jr * NFK + k, s2);
+                            }
+                            // K_il += sum_k D_jk ; K_jl += sum_k D_ik
+#pragma unroll
+                            for (int l = 0; l < NFL; ++l) {
+                                double s1 = 0., s2 = 0.;
+#pragma unroll
+                                for (int k = 0; k < NFK; ++k) {
+                                    s1 = fma(acc[q][k*NFL + l], D[(j0+jr)*nao + k0+k], s1);
+                                    s2 = fma(acc[q][k*NFL + l], D[(i0+ir)*nao + k0+k], s2);
+                                }
+                                if (lds_k) {
+                                    atomicAdd(kacc + ir * a.l_nao + l0 + l - a.l_ao0, s1);
+                                    atomicAdd(kacc + (NFI + jr) * a.l_nao + l0 + l - a.l_ao0, s2);
+                                } else {
+                                    atomicAdd(kil + ir * NFL + l, s1);
+                                    atomicAdd(kjl + jr * NFL + l, s2);
+                                }
+                            }
+                        }
+                    }
+                    wave_sync();
+                    if (!lds_k && active) {
+                        for (int n = tl; n < NFI*NFK + NFJ*NFK + NFI*NFL + NFJ*NFL; n += TS) {
+                            double v = kik[n];
+                            int row, col;
+                            if (n < NFI * NFK) { row = i0 + n / NFK; col = k0 + n % NFK; }
+                            else if (n < NFI * NFK + NFJ * NFK) {
+                                int m = n - NFI * NFK; row = j0 + m / NFK; col = k0 + m % NFK;
+                            } else if (n < NFI * NFK + NFJ * NFK + NFI * NFL) {
+                                int m = n - NFI * NFK - NFJ * NFK; row = i0 + m / NFL; col = l0 + m % NFL;
+                            } else {
+                                int m = n - NFI * NFK - NFJ * NFK - NFI * NFL;
+                                row = j0 + m / NFL; col = l0 + m % NFL;
+                            }
+                            atomic_add(K + (size_t)row * nao + col, v);
+                        }
+                        wave_sync();
+                    }
+                }
+                if (a.vj) {
+                    double *J = a.vj + id * nao2;
+                    for (int n = tl; n < NFKL; n += TS) jkl[n] = 0.;
+                    wave_sync();
+                    if (active) {
+#pragma unroll
+                        for (int q = 0; q < RPL; ++q) {
+                            if (rows[q] < 0) continue;
+                            const int ir = rows[q] / NFJ, jr = rows[q] % NFJ;
+                            const double dij = D[(i0+ir)*nao + j0+jr];
+                            double s = 0.;
 #pragma unroll
                             for (int k = 0; k < NFK; ++k)
 #pragma unroll
                             for (int l = 0; l < NFL; ++l) {
-                                int kx = ck.x[k]*SJ + cl.x[l];
-                                int ky = ck.y[k]*SJ + cl.y[l];
-                                int kz = ck.z[k]*SJ + cl.z[l];
-                                acc[k*NFL + l] = fma(sx[kx] * sy[ky], sz[kz], acc[k*NFL + l]);
+                                s = fma(acc[q][k*NFL + l], D[(k0+k)*nao + l0+l], s);
+                                atomicAdd(jkl + k*NFL + l, acc[q][k*NFL + l] * dij);
                             }
+                            if (a.n_dm == 1) vjij[q] += s;
+                            else atomic_add(J + (i0+ir)*nao + j0+jr, s);
                         }
                     }
+                    wave_sync();
+                    if (active) {
+                        for (int n = tl; n < NFKL; n += TS)
+                            atomic_add(J + (k0 + n / NFL)*nao + l0 + n % NFL, jkl[n]);
+                    }
+                    wave_sync();
                 }
             }
-            // ---- digestion of the team's quartet
-            const int i0 = sp.i0, j0 = sp.j0;
-            const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
-            double *stage = my;       // reuse the team's LDS
-            for (int id = 0; id < a.n_dm; ++id) {
-                const double *D = a.dm + id * nao2;
-                if (a.vj && active && has_row) {
-                    double s = 0.;
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l)
-                        s = fma(acc[k*NFL + l], D[(k0+k)*nao + l0+l], s);
-                    atomic_add(a.vj + id * nao2 + (i0+irow)*nao + j0+jrow, s);
-                }
-                for (int r0 = 0; r0 < NROWS; r0 += S::S_ROWS) {
-                    const int r1 = min(r0 + S::S_ROWS, NROWS);
-                    __syncthreads();
-                    if (active && has_row && row >= r0 && row < r1) {
-#pragma unroll
-                        for (int n = 0; n < NFKL; ++n) stage[(row - r0) * NFKL + n] = acc[n];
-                    }
-                    __syncthreads();
-                    if (!active) continue;
-                    if (a.vj) {
-                        double *J = a.vj + id * nao2;
-                        for (int e = tl; e < NFKL; e += TS) {
-                            int k = e / NFL, l = e - (e / NFL) * NFL;
-                            double s = 0.;
-                            for (int rr = r0; rr < r1; ++rr)
-                                s = fma(stage[(rr - r0) * NFKL + e],
-                                        D[(i0 + rr / NFJ)*nao + j0 + rr % NFJ], s);
-                            atomic_add(J + (k0+k)*nao + l0+l, s);
-                        }
-                    }
-                    if (a.vk) {
-                        double *K = a.vk + id * nao2;
-                        const int ilo = r0 / NFJ, ihi = (r1 - 1) / NFJ;
-                        const int ni = ihi - ilo + 1;
-                        // K_ik (sum over j,l of D_jl) and K_il (sum over j,k of D_jk)
-                        for (int e = tl; e < ni * (NFK + NFL); e += TS) {
-                            int i = ilo + e / (NFK + NFL);
-                            int m = e - (e / (NFK + NFL)) * (NFK + NFL);
-                            double s = 0.;
-                            for (int j = 0; j < NFJ; ++j) {
-                                int rr = i * NFJ + j;
-                                if (rr < r0 || rr >= r1) continue;
-                                const double *srow = stage + (rr - r0) * NFKL;
-                                if (m < NFK) {
-                                    for (int l = 0; l < NFL; ++l)
-                                        s = fma(srow[m*NFL + l], D[(j0+j)*nao + l0+l], s);
-                                } else {
-                                    int l = m - NFK;
-                                    for (int k = 0; k < NFK; ++k)
-                                        s = fma(srow[k*NFL + l], D[(j0+j)*nao + k0+k], s);
-                                }
-                            }
-                            if (m < NFK) atomic_add(K + (i0+i)*nao + k0+m, s);
-                            else atomic_add(K + (i0+i)*nao + l0+m-NFK, s);
-                        }
-                        // K_jk (sum over i,l of D_il) and K_jl (sum over i,k of D_ik)
-                        for (int e = tl; e < NFJ * (NFK + NFL); e += TS) {
-                            int j = e / (NFK + NFL);
-                            int m = e - j * (NFK + NFL);
-                            double s = 0.;
-                            for (int i = ilo; i <= ihi; ++i) {
-                                int rr = i * NFJ + j;
-                                if (rr < r0 || rr >= r1) continue;
-                                const double *srow = stage + (rr - r0) * NFKL;
-                                if (m < NFK) {
-                                    for (int l = 0; l < NFL; ++l)
-                                        s = fma(srow[m*NFL + l], D[(i0+i)*nao + l0+l], s);
-                                } else {
-                                    int l = m - NFK;
-                                    for (int k = 0; k < NFK; ++k)
-                                        s = fma(srow[k*NFL + l], D[(i0+i)*nao + k0+k], s);
-                                }
-                            }
-                            if (m < NFK) atomic_add(K + (j0+j)*nao + k0+m, s);
-                            else atomic_add(K + (j0+j)*nao + l0+m-NFK, s);
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            head += NQ;
+            // drop the NQW processed entries
+            const int nb = qn < NQW ? qn : NQW;
+            uint32_t rest = (nb + lane < qn) ? queue[nb + lane] : 0u;
+            wave_sync();
+            if (nb + lane < qn) queue[lane] = rest;
+            qn -= nb;
+            wave_sync();
         }
-        // move the unprocessed remainder to the front of the queue
-        int rem = qn - head;
-        uint32_t v = 0;
-        if (tid < rem) v = queue[head + tid];
-        __syncthreads();
-        if (tid < rem) queue[tid] = v;
-        qn = rem;
-        __syncthreads();
-        if (flush) break;
+        if (flush && qn == 0) break;
     }
-    if (a.n_quartets && tid == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
+    // flush the open K_ik / K_jk run and the J_ij rows of this team
+    if (a.vk && kacc && cur_k >= 0) {
+        const int kc0 = a.ao_loc[cur_k];
+        for (int n = tl; n < NFI * NFK + NFJ * NFK; n += TS) {
+            double v = kik[n];
+            if (v != 0.) {
+                int row = n < NFI * NFK ? i0 + n / NFK : j0 + (n - NFI * NFK) / NFK;
+                int col = kc0 + (n < NFI * NFK ? n % NFK : (n - NFI * NFK) % NFK);
+                atomic_add(a.vk + (size_t)row * nao + col, v);
+            }
+        }
+    }
+    if (a.vj && a.n_dm == 1) {
+#pragma unroll
+        for (int q = 0; q < RPL; ++q)
+            if (rows[q] >= 0 && vjij[q] != 0.)
+                atomic_add(a.vj + (i0 + rows[q] / NFJ)*nao + j0 + rows[q] % NFJ, vjij[q]);
+    }
+    if (kacc) {
+        __syncthreads();
+        for (int n = threadIdx.x; n < nacc; n += WB * WAVE) {
+            double v = kacc[n];
+            if (v != 0.) {
+                int r = n / a.l_nao, c = n - r * a.l_nao;
+                int row = r < NFI ? i0 + r : j0 + r - NFI;
+                atomic_add(a.vk + (size_t)row * nao + a.l_ao0 + c, v);
+            }
+        }
+    }
+    if (a.n_quartets && lane == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
 }
 
 // ---------------------------------------------------------------- dispatch
+// LDS bytes a workgroup may use on the current device (static + dynamic).
+static size_t lds_limit()
+{
+    static int cached_dev = -1;
+    static size_t cached = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    if (dev != cached_dev) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess
+            || v <= 0)
+            v = 64 * 1024;
+        cached = (size_t)v;
+        cached_dev = dev;
+    }
+    return cached;
+}
+
 template <int LI, int LJ, int LK, int LL>
 struct ClassTraits {
     static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
@@ -883,17 +1093,27 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
         constexpr int NFIJ = ncart(LI) + ncart(LJ);
         size_t lds = (size_t)NFIJ * a.l_nao * sizeof(double);
-        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds <= 64 * 1024;
+        size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * WAVE;
+        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds + lds_static <= lds_limit();
         hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(LANE_WAVES * WAVE),
                            use_lds ? lds : 0, stream, a, kl_chunk, use_lds);
     } else {
         using S = TeamShape<LI, LJ, LK, LL>;
-        const int kl_chunk = 256;
-        int nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
+        int nchunks = (8192 + a.npairs_ij - 1) / a.npairs_ij;
+        int maxchunks = (a.npairs_kl + 127) / 128;
+        if (nchunks > maxchunks) nchunks = maxchunks;
+        if (nchunks < 1) nchunks = 1;
+        const int kl_chunk = (a.npairs_kl + nchunks - 1) / nchunks;
+        nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-        hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB), 0,
-                           stream, a, kl_chunk);
+        constexpr int NFIJ = ncart(LI) + ncart(LJ);
+        size_t lds = (size_t)NFIJ * a.l_nao * sizeof(double);
+        size_t lds_static = sizeof(double) * S::WB * S::PER_WAVE +
+                            sizeof(uint32_t) * S::WB * 2 * WAVE;
+        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds + lds_static <= lds_limit();
+        hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB),
+                           use_lds ? lds : 0, stream, a, kl_chunk, use_lds);
     }
     return hipGetLastError();
 }

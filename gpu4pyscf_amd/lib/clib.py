@@ -14,7 +14,7 @@ _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, 'libjkhip.so')
 TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
 
-JK_ABI_VERSION = 3
+JK_ABI_VERSION = 4
 JK_RYS_NMAX = 11
 JK_LMAX = 3
 
@@ -24,6 +24,7 @@ c_void_p = ctypes.c_void_p
 class JKRysTable(ctypes.Structure):
     _fields_ = [
         ('coef', c_void_p),
+        ('coef32', c_void_p),
         ('off', ctypes.c_int64 * JK_RYS_NMAX),
         ('xmax', ctypes.c_double * JK_RYS_NMAX),
         ('large_r', ctypes.c_int64),
@@ -58,6 +59,8 @@ class JKBuildArgs(ctypes.Structure):
         ('vk', c_void_p),
         ('rys', JKRysTable),
         ('n_quartets', c_void_p),
+        ('fp32_cut', ctypes.c_float),
+        ('n_quartets_fp32', c_void_p),
     ]
 
 
@@ -119,8 +122,10 @@ def rys_table(device):
     if key not in _tables:
         data, meta = load_rys_table_host()
         buf = torch.as_tensor(data, device=device)
+        buf32 = buf.to(torch.float32)
         tab = JKRysTable()
         tab.coef = buf.data_ptr()
+        tab.coef32 = buf32.data_ptr()
         n = len(meta['off'])
         for i in range(JK_RYS_NMAX):
             tab.off[i] = int(meta['off'][i]) if i < n else 0
@@ -128,5 +133,5 @@ def rys_table(device):
         tab.large_r = meta['large_r']
         tab.large_w = meta['large_w']
         tab.degree = meta['degree']
-        _tables[key] = (buf, tab)
+        _tables[key] = ((buf, buf32), tab)
     return _tables[key]

@@ -215,14 +215,16 @@ class _VHFOpt:
         return out
 
     def get_jk(self, dms, hermi, verbose=None, with_j=True, with_k=True,
-               count_quartets=False, time_classes=False):
+               count_quartets=False, time_classes=False, fp32_tol=None):
         '''J/K for density matrices ``dms`` (n_dm, nao_c, nao_c) in the sorted
         Cartesian basis (torch, float64, on the device).
 
         count_quartets: store the surviving quartets of each class task in
         ``self.last_task_counts`` (and their sum in ``self.last_quartets``).
         time_classes: store (task, milliseconds) of each class launch, timed
-        with HIP events on the current stream, in ``self.last_task_ms``.'''
+        with HIP events on the current stream, in ``self.last_task_ms``.
+        fp32_tol: evaluate in FP32 the quartets whose bound
+        sqrt|(ij|ij)| sqrt|(kl|kl)| max|D| is below fp32_tol (None: all FP64).'''
         torch = _torch()
         if self._dev is None:
             self.build()
@@ -250,6 +252,8 @@ class _VHFOpt:
         tasks = self.tasks()
         counter = (torch.zeros(len(tasks), dtype=torch.int64, device=dms.device)
                    if count_quartets else None)
+        counter32 = (torch.zeros(len(tasks), dtype=torch.int64, device=dms.device)
+                     if count_quartets else None)
         events = []
 
         args = clib.JKBuildArgs()
@@ -267,6 +271,7 @@ class _VHFOpt:
         args.vj = vj.data_ptr() if with_j else None
         args.vk = vk.data_ptr() if with_k else None
         args.rys = self._rys
+        args.fp32_cut = np.float32(math.log(fp32_tol)) if fp32_tol else np.float32(-1e30)
         uniq = mol.uniq_l_ctr
         goff = mol.group_offsets
         g_ao0 = mol.ao_loc[goff[:-1]]
@@ -276,6 +281,7 @@ class _VHFOpt:
             args.l_ao0, args.l_nao = int(g_ao0[l]), int(g_nao[l])
             if counter is not None:
                 args.n_quartets = counter.data_ptr() + 8 * t
+                args.n_quartets_fp32 = counter32.data_ptr() + 8 * t
             pij, qij = d['pairs'][i, j], d['q'][i, j]
             pkl, qkl = d['pairs'][k, l], d['q'][k, l]
             args.li, args.lj, args.lk, args.ll = (int(uniq[x, 0]) for x in (i, j, k, l))
@@ -302,6 +308,7 @@ class _VHFOpt:
         if counter is not None:
             self.last_task_counts = dict(zip(tasks, counter.cpu().numpy().tolist()))
             self.last_quartets = int(sum(self.last_task_counts.values()))
+            self.last_quartets_fp32 = int(counter32.sum().item())
         if time_classes:
             torch.cuda.synchronize()
             self.last_task_ms = [(t, e0.elapsed_time(e1)) for t, e0, e1 in events]

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  3
+#define JK_ABI_VERSION  4
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         3       /* max angular momentum on the GPU path */
 
@@ -34,6 +34,7 @@ extern "C" {
  * w_i = large[large_w + n*(n-1)/2 + i] / sqrt(x). */
 typedef struct {
     const double *coef;
+    const float *coef32;        /* coef rounded to float (FP32 quartets) */
     int64_t off[JK_RYS_NMAX];
     double xmax[JK_RYS_NMAX];
     int64_t large_r;            /* offset of large-x nodes in coef */
@@ -70,6 +71,12 @@ typedef struct {
     double *vk;                             /* device or NULL */
     JKRysTable rys;
     unsigned long long *n_quartets;         /* device counter or NULL */
+    /* Mixed precision: a surviving quartet with q_ij + q_kl + max of its six
+     * dm_cond values below fp32_cut is evaluated in FP32 (roots and 2D
+     * recursions; digestion and J/K stay FP64).  -1e30 disables it.  No
+     * counterpart in the reference. */
+    float fp32_cut;
+    unsigned long long *n_quartets_fp32;    /* device counter or NULL */
 } JKBuildArgs;
 
 /* Accumulate one class task into vj/vk (before the hermi post-processing of
