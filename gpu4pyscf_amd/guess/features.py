@@ -1,0 +1,185 @@
+'''Atom-pair descriptors and local frames of the learned initial guess.
+
+Every atom pair (A, B) with R_AB < 6 Angstrom, and every atom with itself,
+gets one row of features:
+
+* one-hot of the block type (same-atom H/C/O, pair HH/HC/HO/CC/CO/OO);
+* 32 Gaussian radial functions of R_AB placed in the slot of its element
+  pair (centres 0.5..6 Angstrom, width 0.25 Angstrom, cosine cutoff 6);
+* the same 32 radial functions summed over the neighbours of A, per
+  neighbour element, and likewise for B (environment of the pair).
+
+The density block of a pair is covariant, so it is expressed in a local frame
+(z from A to B, x towards the nearest other neighbour of A; for a same-atom
+block z and x point to the two nearest neighbours): with Q the frame (rows =
+local axes) and U(Q) its representation on real spherical harmonics,
+D_local = U_A D_AB U_B^T is invariant under rotations of the molecule, and
+D_AB = U_A^T D_local U_B restores the lab frame.
+'''
+
+import numpy as np
+
+from gpu4pyscf_amd.gto.mole import BOHR, ANG_OF
+from gpu4pyscf_amd.gto.cart2sph import c2s, cart_components
+
+ELEMENTS = ('H', 'C', 'O')
+PAIR_TYPES = [(a, b) for i, a in enumerate(ELEMENTS) for b in ELEMENTS[i:]]
+NRBF = 32
+RCUT = 6.0
+N_TYPES = len(ELEMENTS) + len(PAIR_TYPES)          # 3 same-atom + 6 pair types
+FEAT_DIM = N_TYPES + NRBF * len(PAIR_TYPES) + 2 * NRBF * len(ELEMENTS)   # 393
+FEAT_PAD = 400
+OUT_DIM = 196                                        # largest block: O x O in def2-SVP
+
+_centres = np.linspace(0.5, RCUT, NRBF)
+_width = 0.25
+
+
+def rbf(r):
+    r = np.asarray(r, dtype=float)
+    f = np.exp(-(r[..., None] - _centres)**2 / (2 * _width**2))
+    cut = np.where(r < RCUT, .5 * (np.cos(np.pi * r / RCUT) + 1), 0.)
+    return f * cut[..., None]
+
+
+def pair_type(ea, eb, same_atom):
+    if same_atom:
+        return ELEMENTS.index(ea)
+    key = tuple(sorted((ea, eb), key=ELEMENTS.index))
+    return len(ELEMENTS) + PAIR_TYPES.index(key)
+
+
+def _frame(z, x_hint):
+    z = z / np.linalg.norm(z)
+    x = x_hint - np.dot(x_hint, z) * z
+    if np.linalg.norm(x) < 1e-6:
+        t = np.array([1., 0., 0.]) if abs(z[0]) < .9 else np.array([0., 1., 0.])
+        x = t - np.dot(t, z) * z
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    return np.stack([x, y, z])          # rows: local axes in lab coordinates
+
+
+def _sh_rotation(l, Q, _cache={}):
+    '''U with Y_m(Q^T r) = sum_m' Y_m'(r) U[m', m] for PySCF-ordered real
+    harmonics of angular momentum l (p ordered x, y, z).'''
+    if l == 0:
+        return np.ones((1, 1))
+    if l == 1:
+        return Q.copy()
+    key = l
+    if key not in _cache:
+        rng = np.random.default_rng(7)
+        pts = rng.standard_normal((64, 3))
+        _cache[key] = pts / np.linalg.norm(pts, axis=1)[:, None]
+    pts = _cache[key]
+    comps = cart_components(l)
+    C = c2s(l)
+
+    def Y(p):
+        mono = np.stack([p[:, 0]**a * p[:, 1]**b * p[:, 2]**c for a, b, c in comps], axis=1)
+        return mono @ C.T
+    Y0 = Y(pts)
+    Y1 = Y(pts @ Q)        # rows: (Q^T r)^T = r^T Q
+    U, *_ = np.linalg.lstsq(Y0, Y1, rcond=None)
+    return U
+
+
+def ao_rotation(mol, ia, Q):
+    '''AO representation (spherical, PySCF order) of frame rotation Q on the
+    shells of atom ia.'''
+    blocks = []
+    for b in mol._bas[mol._bas[:, 0] == ia]:
+        U = _sh_rotation(int(b[ANG_OF]), Q)
+        for _ in range(int(b[3])):
+            blocks.append(U)
+    n = sum(u.shape[0] for u in blocks)
+    out = np.zeros((n, n))
+    p = 0
+    for u in blocks:
+        k = u.shape[0]
+        out[p:p+k, p:p+k] = u
+        p += k
+    return out
+
+
+def pairs_and_features(mol):
+    '''Rows (A, B, type, Q) and the (n_rows, FEAT_PAD) feature matrix.'''
+    R = mol.atom_coords() * BOHR
+    sym = [mol.atom_symbol(i) for i in range(mol.natm)]
+    for s in sym:
+        if s not in ELEMENTS:
+            raise ValueError(f'learned guess has no parameters for element {s}')
+    n = len(R)
+    D = np.linalg.norm(R[:, None] - R[None], axis=2)
+    env = np.zeros((n, len(ELEMENTS), NRBF))
+    for a in range(n):
+        for c in range(n):
+            if c != a and D[a, c] < RCUT:
+                env[a, ELEMENTS.index(sym[c])] += rbf(D[a, c])
+    rows = []
+    feats = []
+    # neighbours by distance rounded to 1e-6 Angstrom, ties broken by atom
+    # index, so that equal bond lengths give the same frame after rotation
+    Dr = np.round(D, 6)
+    order = np.array([np.lexsort((np.arange(n), Dr[a])) for a in range(n)])
+    for a in range(n):
+        for b in range(a, n):
+            if b != a and D[a, b] >= RCUT:
+                continue
+            same = a == b
+            t = pair_type(sym[a], sym[b], same)
+            f = np.zeros(FEAT_PAD)
+            f[t] = 1.
+            if same:
+                nb = [c for c in order[a] if c != a][:2]
+                z = R[nb[0]] - R[a]
+                x = R[nb[1]] - R[a] if len(nb) > 1 else np.array([1., 0., 0.])
+            else:
+                k = PAIR_TYPES.index(tuple(sorted((sym[a], sym[b]), key=ELEMENTS.index)))
+                f[N_TYPES + k * NRBF:N_TYPES + (k + 1) * NRBF] = rbf(D[a, b])
+                z = R[b] - R[a]
+                nb = [c for c in order[a] if c not in (a, b)]
+                x = R[nb[0]] - R[a] if nb else np.array([1., 0., 0.])
+            off = N_TYPES + NRBF * len(PAIR_TYPES)
+            f[off:off + NRBF * len(ELEMENTS)] = env[a].ravel()
+            f[off + NRBF * len(ELEMENTS):off + 2 * NRBF * len(ELEMENTS)] = env[b].ravel()
+            rows.append((a, b, t, _frame(z, x)))
+            feats.append(f)
+    return rows, np.asarray(feats)
+
+
+def blocks_to_local(mol, dm, rows):
+    '''Targets: the density blocks of ``rows`` in their local frames,
+    (n_rows, OUT_DIM) with a mask of the valid entries.'''
+    aos = mol.aoslice_by_atom()
+    Y = np.zeros((len(rows), OUT_DIM))
+    M = np.zeros((len(rows), OUT_DIM))
+    for r, (a, b, t, Q) in enumerate(rows):
+        Ua = ao_rotation(mol, a, Q)
+        Ub = ao_rotation(mol, b, Q)
+        blk = dm[aos[a, 2]:aos[a, 3], aos[b, 2]:aos[b, 3]]
+        loc = Ua @ blk @ Ub.T
+        k = loc.size
+        if k > OUT_DIM:
+            raise ValueError('density block larger than the network output')
+        Y[r, :k] = loc.ravel()
+        M[r, :k] = 1.
+    return Y, M
+
+
+def local_to_density(mol, rows, Y):
+    '''Assemble the lab-frame density from predicted local blocks.'''
+    aos = mol.aoslice_by_atom()
+    nao = mol.nao
+    dm = np.zeros((nao, nao))
+    for r, (a, b, t, Q) in enumerate(rows):
+        Ua = ao_rotation(mol, a, Q)
+        Ub = ao_rotation(mol, b, Q)
+        na, nb = aos[a, 3] - aos[a, 2], aos[b, 3] - aos[b, 2]
+        loc = Y[r, :na * nb].reshape(na, nb)
+        blk = Ua.T @ loc @ Ub
+        dm[aos[a, 2]:aos[a, 3], aos[b, 2]:aos[b, 3]] = blk
+        if a != b:
+            dm[aos[b, 2]:aos[b, 3], aos[a, 2]:aos[a, 3]] = blk.T
+    return .5 * (dm + dm.T)

@@ -65,7 +65,8 @@ class JKBuildArgs(ctypes.Structure):
 
 
 EXPORTS = ('jk_build', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
-           'jk_int1e', 'jk_last_error', 'jk_abi_version')
+           'jk_int1e', 'mlp_gemm_bf16', 'mlp_silu_backward', 'mlp_colsum', 'mlp_mse_grad',
+           'mlp_adamw', 'mlp_to_bf16', 'jk_last_error', 'jk_abi_version')
 
 _lib = None
 
@@ -89,6 +90,16 @@ def load():
     lib.jk_int1e.restype = ctypes.c_int
     lib.jk_int1e.argtypes = [c_void_p] * 6 + [ctypes.c_int, ctypes.c_int, c_void_p, c_void_p,
                                                ctypes.c_int, JKRysTable, c_void_p]
+    L, I, F, P = ctypes.c_long, ctypes.c_int, ctypes.c_float, c_void_p
+    lib.mlp_gemm_bf16.argtypes = [P, P, P, P, P, I, I, I, L, L, L, L, I, I, P]
+    lib.mlp_silu_backward.argtypes = [P, P, P, P, I, I, P]
+    lib.mlp_colsum.argtypes = [P, P, I, I, P]
+    lib.mlp_mse_grad.argtypes = [P, P, P, P, P, P, L, F, P]
+    lib.mlp_adamw.argtypes = [P, P, P, P, P, L, F, F, F, F, F, I, P]
+    lib.mlp_to_bf16.argtypes = [P, P, L, P]
+    for name in ('mlp_gemm_bf16', 'mlp_silu_backward', 'mlp_colsum', 'mlp_mse_grad',
+                 'mlp_adamw', 'mlp_to_bf16'):
+        getattr(lib, name).restype = ctypes.c_int
     if lib.jk_abi_version() != JK_ABI_VERSION:
         raise RuntimeError('libjkhip.so ABI version mismatch; rebuild it')
     _lib = lib

@@ -98,6 +98,11 @@ class RHF:
     diis_start_cycle = 1
     direct_scf_tol = 1e-13
     init_guess = 'atom'
+    # mixed precision: quartets with sqrt|(ij|ij)(kl|kl)| max|dD| below
+    # fp32_tol run in FP32 (None: plain FP64); the converged energy is then
+    # re-evaluated from one FP64 build of the full density (final_fp64)
+    fp32_tol = None
+    final_fp64 = True
 
     def __init__(self, mol):
         self.mol = mol
@@ -108,6 +113,9 @@ class RHF:
         self.cycles = 0
         self.mo_energy = self.mo_coeff = self.mo_occ = None
         self.jk_builds = 0
+        self.quartets = 0
+        self.quartets_fp32 = 0
+        self._fp32_now = None
         self.timings = {}
         self.history = []
         self.jk_hook = None
@@ -130,7 +138,10 @@ class RHF:
             return self.jk_hook(dm, hermi)
         opt = self.opt
         dms = opt.apply_coeff_C_mat_CT(dm.reshape(-1, dm.shape[-1], dm.shape[-1]))
-        vj, vk = opt.get_jk(dms, hermi, with_j=with_j, with_k=with_k)
+        vj, vk = opt.get_jk(dms, hermi, with_j=with_j, with_k=with_k,
+                            fp32_tol=self._fp32_now, count_quartets=True)
+        self.quartets += opt.last_quartets
+        self.quartets_fp32 += opt.last_quartets_fp32
         shape = dm.shape
         vj = opt.apply_coeff_CT_mat_C(vj).reshape(shape) if with_j else None
         vk = opt.apply_coeff_CT_mat_C(vk).reshape(shape) if with_k else None
@@ -173,8 +184,13 @@ class RHF:
         torch = _torch()
         key = self.init_guess if key is None else key
         if callable(key):
-            dm = key(self.mol)
-            return torch.as_tensor(dm, dtype=torch.float64, device=S.device)
+            # a learned or user guess: symmetrise and scale to Tr(DS) = N
+            dm = torch.as_tensor(key(self.mol), dtype=torch.float64, device=S.device)
+            dm = .5 * (dm + dm.T)
+            tr = float(torch.sum(dm * S))
+            if tr > 0:
+                dm = dm * (self.mol.nelectron / tr)
+            return dm
         if not isinstance(key, str):
             return torch.as_tensor(key, dtype=torch.float64, device=S.device)
         if key == '1e':
@@ -196,7 +212,9 @@ class RHF:
         conv_tol_grad = self.conv_tol_grad or math.sqrt(self.conv_tol)
         dm = self.get_init_guess(h1e, S, X, dm0 if dm0 is not None else None)
         t_guess = time.perf_counter()
+        self._fp32_now = None           # the full-density build is FP64
         vhf = self.get_veff(dm)
+        self._fp32_now = self.fp32_tol  # incremental builds may use FP32
         e_tot = float(torch.sum(dm * (h1e + .5 * vhf))) + e_nuc
         diis = CDIIS(self.diis_space)
         self.converged = False
@@ -229,6 +247,11 @@ class RHF:
                 self.converged = True
                 break
         self.cycles = cycle + 1
+        if self.fp32_tol and self.final_fp64:
+            self._fp32_now = None
+            vhf = self.get_veff(dm)
+            e_tot = float(torch.sum(dm * (h1e + .5 * vhf))) + e_nuc
+        self._fp32_now = None
         self.e_tot = e_tot
         self.mo_energy, self.mo_coeff, self.mo_occ = mo_energy, mo_coeff, mo_occ
         self.dm = dm

@@ -128,6 +128,36 @@ int jk_int1e(double *S, double *T, double *V, const int32_t *bas, const double *
              const int32_t *ao_loc, int nbas, int nao, const double *coords,
              const double *charges, int natm, JKRysTable rys, void *stream);
 
+/* ---- learned initial guess: BF16 MFMA network kernels (no counterpart in
+ * the reference; they replace its minao/atom guess, gpu4pyscf/scf/hf.py:
+ * 414-607).  bf16 buffers are uint16_t bit patterns; accumulation is FP32. */
+
+/* C(m,n) = sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn];
+ * mode 0: C = acc + bias; mode 1: C = acc + bias, act = bf16(silu(C));
+ * mode 2: C += acc.  bias may be NULL. */
+int mlp_gemm_bf16(float *C, uint16_t *act, const uint16_t *A, const uint16_t *B,
+                  const float *bias, int M, int N, int K, long sam, long sak, long sbk,
+                  long sbn, int ldc, int mode, void *stream);
+
+/* dZ = dA * silu'(Z) as bf16, db[n] += sum_m dZ(m,n); M x N row major. */
+int mlp_silu_backward(uint16_t *dz, float *db, const float *dA, const float *Z, int M, int N,
+                      void *stream);
+
+/* db[n] += sum_m G(m,n). */
+int mlp_colsum(float *db, const float *G, int M, int N, void *stream);
+
+/* loss += scale sum mask (Y-T)^2; dy = 2 scale mask (Y-T) (FP32 and bf16). */
+int mlp_mse_grad(float *dy, uint16_t *dy_bf16, float *loss, const float *Y, const float *T,
+                 const float *mask, long n, float scale, void *stream);
+
+/* AdamW step on FP32 master weights (bias-corrected, decoupled weight
+ * decay), refreshing the bf16 copy. */
+int mlp_adamw(float *w, uint16_t *wb, float *m, float *v, const float *g, long n, float lr,
+              float beta1, float beta2, float eps, float wd, int step, void *stream);
+
+/* bf16(in), round to nearest even. */
+int mlp_to_bf16(uint16_t *out, const float *in, long n, void *stream);
+
 /* Last error message of this thread ("" if none). */
 const char *jk_last_error(void);
 

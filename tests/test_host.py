@@ -15,7 +15,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared_symbols():
     text = open(os.path.join(ROOT, 'include', 'jkhip.h')).read()
-    return sorted(set(re.findall(r'^\s*(?:int|const char \*)\s*(jk_\w+)\s*\(', text, re.M)))
+    return sorted(set(re.findall(r'^\s*(?:int|const char \*)\s*((?:jk|mlp)_\w+)\s*\(',
+                                 text, re.M)))
 
 
 def test_library_exports_header_symbols():

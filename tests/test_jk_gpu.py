@@ -67,3 +67,27 @@ def test_water8_svp_jk_and_quartet_set():
     assert opt.last_quartets == nsurv
     assert abs(vj.cpu().numpy() - rj).max() < 1e-10
     assert abs(vk.cpu().numpy() - rk).max() < 1e-10
+
+
+def test_mixed_precision_incremental_jk():
+    '''FP32 quartets on an incremental density change: error against the
+    FP64 build stays below the bound implied by the threshold, and a large
+    fraction of quartets runs in FP32.'''
+    import torch
+    from gpu4pyscf_amd.scf import jk
+    mol, dm = _water8_svp()
+    opt = jk._VHFOpt(mol).build()
+    rng = np.random.default_rng(5)
+    g = rng.standard_normal((mol.nao, mol.nao))
+    ddm = torch.as_tensor(1e-3 * (g + g.T) / 2, device='cuda')
+    dms = opt.apply_coeff_C_mat_CT(ddm[None])
+    vj0, vk0 = opt.get_jk(dms, 1, count_quartets=True)
+    n0 = opt.last_quartets
+    assert opt.last_quartets_fp32 == 0
+    vj1, vk1 = opt.get_jk(dms, 1, count_quartets=True, fp32_tol=1e-6)
+    assert opt.last_quartets == n0
+    frac = opt.last_quartets_fp32 / n0
+    assert frac > 0.2
+    err = max(float((vj1 - vj0).abs().max()), float((vk1 - vk0).abs().max()))
+    print(f'fp32 fraction {frac:.3f}, max |dJ|,|dK| = {err:.2e}')
+    assert err < 1e-9

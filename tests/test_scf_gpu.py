@@ -75,3 +75,23 @@ def test_rhf_water_matches_oracle():
     assert mf.converged
     e_ref, _ = scf_oracle.rhf(mol, conv_tol=1e-10)
     assert abs(e - e_ref) < 1e-9
+
+
+def test_rhf_mixed_precision_water_cluster():
+    '''Mixed FP32/FP64 incremental builds reach the FP64 energy within 1e-6.'''
+    from gpu4pyscf_amd.gto import geometry as G
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.scf.hf import RHF
+    mol = M(atom=G.to_string(G.water_cluster((2, 2, 1), 3.0, seed=0)), basis='def2-svp')
+    mf = RHF(mol)
+    mf.conv_tol = 1e-9
+    e64 = mf.kernel()
+    mf2 = RHF(mol)
+    mf2.conv_tol = 1e-9
+    mf2.fp32_tol = 1e-6
+    e32 = mf2.kernel()
+    assert mf.converged and mf2.converged
+    print(f'mixed: |dE| = {abs(e32 - e64):.2e}, fp32 quartets '
+          f'{mf2.quartets_fp32 / mf2.quartets:.3f}')
+    assert abs(e32 - e64) < 1e-6
+    assert mf2.quartets_fp32 > 0
