@@ -54,6 +54,44 @@ def cpu_baseline(opt, dms_c, target_s=15.):
                        f'get_jk ({n} quartets, {t:.1f} s)')
 
 
+def scf_leg(mol, fp32_tol, guess_model=None, conv_tol=1e-8):
+    '''Direct RHF to conv_tol: plain FP64 against mixed FP32/FP64 (and the
+    learned guess when a model is given); wall times include everything
+    after the screening setup.'''
+    import torch
+    from gpu4pyscf_amd.scf.hf import RHF
+    res = {}
+    opt = None
+    variants = [('fp64', None, 'atom'), ('mixed', fp32_tol, 'atom')]
+    if guess_model:
+        from gpu4pyscf_amd.guess.learned import LearnedGuess
+        variants.append(('learned_mixed', fp32_tol, LearnedGuess.load(guess_model)))
+    for name, tol, guess in variants:
+        mf = RHF(mol)
+        mf.conv_tol = conv_tol
+        mf.fp32_tol = tol
+        mf.init_guess = guess
+        if opt is not None:
+            mf._opt = opt
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e = mf.kernel()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        opt = mf._opt
+        res[name] = dict(e_tot=e, wall_s=dt, cycles=mf.cycles, converged=mf.converged,
+                         jk_builds=mf.jk_builds, quartets=mf.quartets,
+                         fp32_fraction=mf.quartets_fp32 / max(mf.quartets, 1))
+        log(f'[bench] SCF {name}: E={e:.10f} {dt:.2f} s, {mf.cycles} cycles, '
+            f'fp32 {res[name]["fp32_fraction"]:.3f}')
+    e0 = res['fp64']['e_tot']
+    for name in res:
+        res[name]['abs_dE_vs_fp64'] = abs(res[name]['e_tot'] - e0)
+    res['conv_tol'] = conv_tol
+    res['fp32_tol'] = fp32_tol
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -61,6 +99,10 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--config', default='water32-tzvp')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-scf', action='store_true')
+    ap.add_argument('--fp32-tol', type=float, default=1e-6)
+    ap.add_argument('--guess-model', default=None,
+                    help='learned-guess weights (.npz) for the learned+mixed SCF')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported',
@@ -148,6 +190,8 @@ def main():
                                     f'nao={mol.nao}, quartets={nq}',
                            global_batch=1, seq_len=0, parallelism='single GPU'),
                roofline=roofline, setup_s=t_build)
+    if not args.no_scf:
+        out['scf'] = scf_leg(mol, args.fp32_tol, args.guess_model)
     if not args.no_cpu:
         log('[bench] CPU baseline (oracle) ...')
         out['cpu_baseline'] = cpu_baseline(opt, dms_c)

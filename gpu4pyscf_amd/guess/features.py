@@ -29,7 +29,7 @@ RCUT = 6.0
 N_TYPES = len(ELEMENTS) + len(PAIR_TYPES)          # 3 same-atom + 6 pair types
 FEAT_DIM = N_TYPES + NRBF * len(PAIR_TYPES) + 2 * NRBF * len(ELEMENTS)   # 393
 FEAT_PAD = 400
-OUT_DIM = 196                                        # largest block: O x O in def2-SVP
+OUT_DIM = 196                                        # default: O x O block of def2-SVP
 
 _centres = np.linspace(0.5, RCUT, NRBF)
 _width = 0.25
@@ -149,19 +149,26 @@ def pairs_and_features(mol):
     return rows, np.asarray(feats)
 
 
-def blocks_to_local(mol, dm, rows):
-    '''Targets: the density blocks of ``rows`` in their local frames,
-    (n_rows, OUT_DIM) with a mask of the valid entries.'''
+def max_block(mol):
+    '''Largest atom-pair density block of the molecule's basis.'''
     aos = mol.aoslice_by_atom()
-    Y = np.zeros((len(rows), OUT_DIM))
-    M = np.zeros((len(rows), OUT_DIM))
+    n = (aos[:, 3] - aos[:, 2]).max()
+    return int(n * n)
+
+
+def blocks_to_local(mol, dm, rows, out_dim=OUT_DIM):
+    '''Targets: the density blocks of ``rows`` in their local frames,
+    (n_rows, out_dim) with a mask of the valid entries.'''
+    aos = mol.aoslice_by_atom()
+    Y = np.zeros((len(rows), out_dim))
+    M = np.zeros((len(rows), out_dim))
     for r, (a, b, t, Q) in enumerate(rows):
         Ua = ao_rotation(mol, a, Q)
         Ub = ao_rotation(mol, b, Q)
         blk = dm[aos[a, 2]:aos[a, 3], aos[b, 2]:aos[b, 3]]
         loc = Ua @ blk @ Ub.T
         k = loc.size
-        if k > OUT_DIM:
+        if k > out_dim:
             raise ValueError('density block larger than the network output')
         Y[r, :k] = loc.ravel()
         M[r, :k] = 1.

@@ -16,7 +16,9 @@ import time
 import numpy as np
 
 from gpu4pyscf_amd.guess import features as F
-from gpu4pyscf_amd.guess.mlp import PairMLP, DIMS, to_bf16
+from gpu4pyscf_amd.guess.mlp import PairMLP, to_bf16
+
+HIDDEN = (256, 256, 256)
 
 
 def _torch():
@@ -44,11 +46,12 @@ def make_dataset(seeds, shape=(2, 2, 2), spacing=3.0, basis='def2-svp', log=None
     '''Rows (X, Y, mask) from converged densities of seeded clusters.'''
     Xs, Ys, Ms = [], [], []
     t0 = time.perf_counter()
+    out_dim = F.max_block(water_mol(seeds[0], shape, spacing, basis))
     for n, s in enumerate(seeds):
         mol = water_mol(s, shape, spacing, basis)
         dm, _ = converged_density(mol)
         rows, X = F.pairs_and_features(mol)
-        Y, Mk = F.blocks_to_local(mol, dm, rows)
+        Y, Mk = F.blocks_to_local(mol, dm, rows, out_dim)
         Xs.append(X); Ys.append(Y); Ms.append(Mk)
         if log and (n + 1) % 25 == 0:
             log(f'[dataset] {n+1}/{len(seeds)} geometries, {time.perf_counter()-t0:.1f} s')
@@ -61,7 +64,7 @@ def train(data, epochs=50, lr=1e-3, batch=4096, seed=0, device='cuda', log=None,
     '''Train a PairMLP on (X, Y, mask); returns (net, loss history).'''
     torch = _torch()
     X, Y, Mk = data
-    net = PairMLP(device, seed=seed, dims=DIMS)
+    net = PairMLP(device, seed=seed, dims=(X.shape[1],) + HIDDEN + (Y.shape[1],))
     xb = to_bf16(X, device)
     yt = torch.as_tensor(Y, device=device)
     mt = torch.as_tensor(Mk, device=device)
