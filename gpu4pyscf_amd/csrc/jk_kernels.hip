@@ -677,12 +677,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
 }
 
 // ---------------------------------------------------------------- team
-// Larger classes: teams of TS lanes inside one wavefront evaluate one quartet
-// each; every wavefront of the block scans and evaluates its own kl
-// candidates, so only LDS ordering within a wavefront is needed.
-// Per primitive quartet: lanes r < N evaluate one Rys root each, lanes
-// t < 3N build the 2D table of (root t/3, direction t%3) into LDS, then each
-// lane owns one or two (i,j) rows of the block and accumulates all (k,l).
+// Team of TS lanes per quartet, NQ quartets per block of TB threads.
 template <int LI, int LJ, int LK, int LL>
 struct TeamShape {
     static constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
@@ -690,138 +685,114 @@ struct TeamShape {
     static constexpr int NFKL = NFK * NFL;
     static constexpr int N = (LI + LJ + LK + LL) / 2 + 1;
     static constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
-    static constexpr int RPL = NROWS > WAVE ? 2 : 1;                 // rows per lane
-    static constexpr int TNEED0 = (NROWS + RPL - 1) / RPL;
-    static constexpr int TNEED = TNEED0 > 3 * N ? TNEED0 : 3 * N;
-    static constexpr int NQW = TNEED <= 16 ? 4 : (TNEED <= 32 ? 2 : 1);   // teams per wave
-    static constexpr int TS = WAVE / NQW;
-    // per-team LDS (doubles): roots, 2D tables, J_kl, K_ik, K_jk, K_il, K_jl
-    static constexpr int O_RW = 0;
-    static constexpr int O_G = 2 * N;
-    static constexpr int O_JKL = O_G + N * 3 * GS;
-    static constexpr int O_KIK = O_JKL + NFKL;
-    static constexpr int O_KJK = O_KIK + NFI * NFK;
-    static constexpr int O_KIL = O_KJK + NFJ * NFK;
-    static constexpr int O_KJL = O_KIL + NFI * NFL;
-    static constexpr int PER_TEAM = O_KJL + NFJ * NFL;
-    static constexpr int PER_WAVE = NQW * PER_TEAM;
-    // waves per block within ~47 KB of static LDS
-    static constexpr int WB0 = 6000 / PER_WAVE;
-    static constexpr int WB = WB0 >= 4 ? 4 : (WB0 >= 2 ? 2 : 1);
-    static constexpr int TB = WB * WAVE;
-    static_assert(WB * PER_WAVE * 8 + WB * 2 * WAVE * 4 <= 48 * 1024,
-                  "team kernel static LDS above 48 KB");
+    static constexpr int TS0 = NROWS > 3 * N ? NROWS : 3 * N;
+    static constexpr int TB = TS0 <= 64 ? 64 : (TS0 <= 128 ? 128 : 256);
+    static constexpr int G_DBL = 2 * N + N * 3 * GS;            // roots + 2D integrals
+    static constexpr int S_ROWS0 = 4096 / NFKL;                 // rows staged per pass
+    static constexpr int S_ROWS = S_ROWS0 < NROWS ? (S_ROWS0 > 0 ? S_ROWS0 : 1) : NROWS;
+    static constexpr int S_DBL = S_ROWS * NFKL;
+    static constexpr int PER_Q = G_DBL > S_DBL ? G_DBL : S_DBL;
+    static constexpr int NQ0 = TB / TS0;
+    static constexpr int NQ1 = (6144 / PER_Q) > 0 ? (6144 / PER_Q) : 1;   // 48 KB of LDS
+    static constexpr int NQ = NQ0 < NQ1 ? NQ0 : NQ1;
+    static constexpr int TS = TB / NQ;
 };
 
 template <int LI, int LJ, int LK, int LL>
 __global__ __launch_bounds__((TeamShape<LI, LJ, LK, LL>::TB))
-void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
+void jk_team_kernel(JKBuildArgs a, int kl_chunk)
 {
     using S = TeamShape<LI, LJ, LK, LL>;
     constexpr int NFI = S::NFI, NFJ = S::NFJ, NFK = S::NFK, NFL = S::NFL;
     constexpr int NROWS = S::NROWS, NFKL = S::NFKL, N = S::N, GS = S::GS;
-    constexpr int RPL = S::RPL, NQW = S::NQW, TS = S::TS, WB = S::WB;
+    constexpr int TB = S::TB, NQ = S::NQ, TS = S::TS;
+    constexpr int NW = TB / WAVE;
     constexpr CartPow<LI> ci{};
     constexpr CartPow<LJ> cj{};
     constexpr CartPow<LK> ck{};
     constexpr CartPow<LL> cl{};
     constexpr int SJ = LL+1, SK = (LK+1)*(LL+1), SI = (LJ+1)*(LK+1)*(LL+1);
 
-    __shared__ uint32_t queue_all[WB][2 * WAVE];
-    __shared__ double lds[WB * S::PER_WAVE];
-    extern __shared__ double kacc_dyn[];
+    __shared__ uint32_t queue[2 * TB];
+    __shared__ int wave_cnt[NW + 1];
+    __shared__ int stop_flag;
+    __shared__ double lds[NQ * S::PER_Q];
 
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wid = threadIdx.x / WAVE;
-    const int team = lane / TS;
-    const int tl = lane - team * TS;
-    uint32_t *queue = queue_all[wid];
-    double *my = lds + wid * S::PER_WAVE + team * S::PER_TEAM;
-    double *rw = my + S::O_RW, *gbuf = my + S::O_G, *jkl = my + S::O_JKL;
-    double *kik = my + S::O_KIK, *kjk = my + S::O_KJK;
-    double *kil = my + S::O_KIL, *kjl = my + S::O_KJL;
-
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wid = tid / WAVE;
+    const int team = tid / TS;
+    const int tl = tid - team * TS;
     const int nbas = a.nbas, nao = a.nao;
+    const size_t nao2 = (size_t)nao * nao;
     WorkItem wi = work_item(a.npairs_kl, kl_chunk);
     const uint32_t bas_ij = a.pairs_ij[wi.pij];
     const float kl_cutoff = a.cutoff - a.q_ij[wi.pij];
     if (a.q_kl[wi.kl0] + a.dm_penalty + Q_COND_MARGIN < kl_cutoff) return;
     ShellPair sp;
     load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
-    const size_t nao2 = (size_t)nao * nao;
-    const int i0 = sp.i0, j0 = sp.j0;
-    double *kacc = use_lds ? kacc_dyn : nullptr;      // K_il / K_jl rows (one density)
-    const int nacc = (NFI + NFJ) * a.l_nao;
-    if (kacc) {
-        for (int n = threadIdx.x; n < nacc; n += WB * WAVE) kacc[n] = 0.;
-    }
-    __syncthreads();
-
-    // rows owned by this lane
-    int rows[RPL];
-    int rpow[RPL][6];
-#pragma unroll
-    for (int q = 0; q < RPL; ++q) {
-        int row = tl + q * TS;
-        rows[q] = row < NROWS ? row : -1;
-        int ir = row < NROWS ? row / NFJ : 0, jr = row < NROWS ? row % NFJ : 0;
-        int px = 0, py = 0, pz = 0, qx = 0, qy = 0, qz = 0;
-#pragma unroll
-        for (int i = 0; i < NFI; ++i)
-            if (i == ir) { px = ci.x[i]; py = ci.y[i]; pz = ci.z[i]; }
-#pragma unroll
-        for (int j = 0; j < NFJ; ++j)
-            if (j == jr) { qx = cj.x[j]; qy = cj.y[j]; qz = cj.z[j]; }
-        rpow[q][0] = px; rpow[q][1] = py; rpow[q][2] = pz;
-        rpow[q][3] = qx; rpow[q][4] = qy; rpow[q][5] = qz;
-    }
-    double vjij[RPL];
-#pragma unroll
-    for (int q = 0; q < RPL; ++q) vjij[q] = 0.;
-    int cur_k = -1;            // k shell of the K_ik / K_jk run held in LDS
-    for (int n = tl; n < NFI * NFK + NFJ * NFK; n += TS) kik[n] = 0.;   // kik, kjk adjacent
-
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    double *my = lds + (team < NQ ? team : 0) * S::PER_Q;
+    double *rw = my;                  // [2N]
+    double *gbuf = my + 2 * N;        // [N][3][GS]
+    const int row = tl;               // (i, j) row owned by this lane
+    const bool has_row = team < NQ && row < NROWS;
+    const int irow = has_row ? row / NFJ : 0, jrow = has_row ? row % NFJ : 0;
     int qn = 0;
     unsigned long long nsurv = 0;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     bool stop = false;
-    for (int base = wi.kl0 + wid * WAVE; base < wi.kl1 || qn > 0; base += WB * WAVE) {
+    for (int base = wi.kl0; ; base += TB) {
         if (base < wi.kl1 && !stop) {
-            int pkl = base + lane;
+            int pkl = base + tid;
             bool keep = false, fail_margin = false;
+            uint32_t bkl = 0;
             if (pkl < wi.kl1) {
                 float q_kl = a.q_kl[pkl];
+                bkl = a.pairs_kl[pkl];
                 fail_margin = q_kl + a.dm_penalty + Q_COND_MARGIN < kl_cutoff;
-                keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, a.pairs_kl[pkl],
+                keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
                                 sp.ish, sp.jsh, nbas, a.dm_cond);
             }
             uint64_t m = __ballot(keep);
-            if (keep) queue[qn + __popcll(m & lt_mask)] = pkl;
-            qn += __popcll(m);
-            nsurv += __popcll(m);
-            stop = __ballot(fail_margin) != 0;
-            wave_sync();
+            uint64_t mf = __ballot(fail_margin);
+            if (tid == 0) stop_flag = 0;
+            __syncthreads();
+            if (lane == 0) {
+                wave_cnt[wid] = __popcll(m);
+                if (mf) stop_flag = 1;
+            }
+            __syncthreads();
+            int off = qn;
+            for (int w = 0; w < wid; ++w) off += wave_cnt[w];
+            if (keep) queue[off + __popcll(m & lt_mask)] = pkl;
+            int tot = 0;
+            for (int w = 0; w < NW; ++w) tot += wave_cnt[w];
+            qn += tot;
+            nsurv += tot;
+            stop = stop_flag != 0;
+            __syncthreads();
         }
-        const bool flush = (base + WB * WAVE >= wi.kl1) || stop;
-        while (qn >= NQW || (flush && qn > 0)) {
-            const bool active = team < qn;
-            const int pkl = queue[active ? team : 0];
-            const uint32_t bas_kl = a.pairs_kl[pkl];
-            const int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
+        bool flush = base + TB >= wi.kl1 || stop;
+        int head = 0;
+        while (qn - head >= NQ || (flush && head < qn)) {
+            // team `team` evaluates queue[head + team]
+            const int qidx = head + team;
+            const bool active = team < NQ && qidx < qn;
+            const int pkl = queue[active ? qidx : head];
+            uint32_t bas_kl = a.pairs_kl[pkl];
+            int ksh = bas_kl / nbas, lsh = bas_kl - (bas_kl / nbas) * nbas;
             double fac_sym = 1.;
             if (sp.ish == sp.jsh) fac_sym *= .5;
             if (ksh == lsh) fac_sym *= .5;
             if (bas_ij == bas_kl) fac_sym *= .5;
             const double *rk = a.env + a.bas[ksh * 8 + 7], *rl = a.env + a.bas[lsh * 8 + 7];
-            const double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
+            double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
             const int nij = a.iprim * a.jprim, nkl = a.kprim * a.lprim;
             const double *pdij = a.pdata_ij + (size_t)wi.pij * nij * PREC;
             const double *pdkl = a.pdata_kl + (size_t)pkl * nkl * PREC;
-            double acc[RPL][NFKL];
+            double acc[NFKL];
 #pragma unroll
-            for (int q = 0; q < RPL; ++q)
-#pragma unroll
-                for (int n = 0; n < NFKL; ++n) acc[q][n] = 0.;
+            for (int n = 0; n < NFKL; ++n) acc[n] = 0.;
             for (int kp = 0; kp < nkl; ++kp) {
                 const double *qr = pdkl + kp * PREC;
                 const double akl = qr[0], rakl = qr[1];
@@ -837,233 +808,186 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                         PQ[d] = pr[2+d] - Q[d];
                         QC[d] = Q[d] - rk[d];
                     }
-                    const double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
-                    const double r_aa = 1. / (aij + akl);
-                    const double theta = aij * akl * r_aa;
-                    const double fac = pr[5] * Kkl * sqrt(r_aa);
-                    if (active && tl < N) {
-                        double r, w;
-                        rys_root_one<N>(a.rys, theta * rr, tl, r, w);
-                        rw[2*tl] = r; rw[2*tl+1] = w;
+                    double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
+                    double r_aa = 1. / (aij + akl);
+                    double theta = aij * akl * r_aa;
+                    double fac = pr[5] * Kkl * sqrt(r_aa);
+                    __syncthreads();
+                    if (active && tl == 0) {
+                        double rt[N], wt[N];
+                        rys_roots<N>(a.rys, theta * rr, rt, wt);
+#pragma unroll
+                        for (int r = 0; r < N; ++r) { rw[2*r] = rt[r]; rw[2*r+1] = wt[r]; }
                     }
-                    wave_sync();
+                    __syncthreads();
                     if (active && tl < 3 * N) {
-                        const int r = tl / 3, d = tl - (tl / 3) * 3;
-                        const double rt_aa = rw[2*r] * r_aa;
-                        const double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
-                        const double b10 = .5 * raij * (1. - rt_aij);
-                        const double b01 = .5 * rakl * (1. - rt_akl);
-                        const double b00 = .5 * rt_aa;
-                        const double pa = d == 0 ? PA[0] : (d == 1 ? PA[1] : PA[2]);
-                        const double pq = d == 0 ? PQ[0] : (d == 1 ? PQ[1] : PQ[2]);
-                        const double qc = d == 0 ? QC[0] : (d == 1 ? QC[1] : QC[2]);
-                        const double rd = d == 0 ? sp.rij[0] : (d == 1 ? sp.rij[1] : sp.rij[2]);
-                        const double kd = d == 0 ? rkl[0] : (d == 1 ? rkl[1] : rkl[2]);
-                        const double g00 = d == 2 ? rw[2*r+1] * fac : 1.;
+                        int r = tl / 3, d = tl - (tl / 3) * 3;
+                        double rt_aa = rw[2*r] * r_aa;
+                        double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
+                        double b10 = .5 * raij * (1. - rt_aij);
+                        double b01 = .5 * rakl * (1. - rt_akl);
+                        double b00 = .5 * rt_aa;
                         double g[GS];
+                        double pa = d == 0 ? PA[0] : (d == 1 ? PA[1] : PA[2]);
+                        double pq = d == 0 ? PQ[0] : (d == 1 ? PQ[1] : PQ[2]);
+                        double qc = d == 0 ? QC[0] : (d == 1 ? QC[1] : QC[2]);
+                        double rd = d == 0 ? sp.rij[0] : (d == 1 ? sp.rij[1] : sp.rij[2]);
+                        double kd = d == 0 ? rkl[0] : (d == 1 ? rkl[1] : rkl[2]);
+                        double g00 = d == 2 ? rw[2*r+1] * fac : 1.;
                         build_g<LI, LJ, LK, LL, double>(g, g00, pa - rt_aij * pq,
-                                                        qc + rt_akl * pq, b10, b01, b00, rd, kd);
+                                                qc + rt_akl * pq,
+                                                b10, b01, b00, rd, kd);
                         double *dst = gbuf + (r * 3 + d) * GS;
 #pragma unroll
                         for (int n = 0; n < GS; ++n) dst[n] = g[n];
                     }
-                    wave_sync();
-                    if (active) {
+                    __syncthreads();
+                    if (active && has_row) {
+                        int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+                        // Cartesian powers of this row's i and j (runtime lookup)
 #pragma unroll
-                        for (int q = 0; q < RPL; ++q) {
-                            if (rows[q] < 0) continue;
-                            const int ox = rpow[q][0] * SI + rpow[q][3] * SK;
-                            const int oy = rpow[q][1] * SI + rpow[q][4] * SK;
-                            const int oz = rpow[q][2] * SI + rpow[q][5] * SK;
-                            for (int r = 0; r < N; ++r) {
-                                const double *gx = gbuf + (r * 3 + 0) * GS + ox;
-                                const double *gy = gbuf + (r * 3 + 1) * GS + oy;
-                                const double *gz = gbuf + (r * 3 + 2) * GS + oz;
-                                double sx[SK], sy[SK], sz[SK];
+                        for (int i = 0; i < NFI; ++i)
+                            if (i == irow) { ax = ci.x[i]; ay = ci.y[i]; az = ci.z[i]; }
 #pragma unroll
-                                for (int n = 0; n < SK; ++n) { sx[n] = gx[n]; sy[n] = gy[n]; sz[n] = gz[n]; }
+                        for (int j = 0; j < NFJ; ++j)
+                            if (j == jrow) { bx = cj.x[j]; by = cj.y[j]; bz = cj.z[j]; }
+                        const int ox = ax * SI + bx * SK, oy = ay * SI + by * SK, oz = az * SI + bz * SK;
+                        for (int r = 0; r < N; ++r) {
+                            const double *gx = gbuf + (r * 3 + 0) * GS + ox;
+                            const double *gy = gbuf + (r * 3 + 1) * GS + oy;
+                            const double *gz = gbuf + (r * 3 + 2) * GS + oz;
+                            double sx[SK], sy[SK], sz[SK];
 #pragma unroll
-                                for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                                for (int l = 0; l < NFL; ++l) {
-                                    const int kx = ck.x[k]*SJ + cl.x[l];
-                                    const int ky = ck.y[k]*SJ + cl.y[l];
-                                    const int kz = ck.z[k]*SJ + cl.z[l];
-                                    acc[q][k*NFL + l] = fma(sx[kx] * sy[ky], sz[kz], acc[q][k*NFL + l]);
-                                }
-                            }
-                        }
-                    }
-                    wave_sync();
-                }
-            }
-            // ---- digestion (one density through LDS, else direct atomics)
-            const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
-            for (int id = 0; id < a.n_dm; ++id) {
-                const double *D = a.dm + id * nao2;
-                const bool lds_k = kacc != nullptr;      // n_dm == 1 when set
-                if (a.vk) {
-                    double *K = a.vk + id * nao2;
-                    // a new k shell ends the K_ik / K_jk run: flush it
-                    const int run_k = active ? ksh : cur_k;
-                    if (lds_k && run_k != cur_k && cur_k >= 0) {
-                        const int kc0 = a.ao_loc[cur_k];
-                        for (int n = tl; n < NFI * NFK + NFJ * NFK; n += TS) {
-                            double v = kik[n];
-                            if (v != 0.) {
-                                int row = n < NFI * NFK ? i0 + n / NFK : j0 + (n - NFI * NFK) / NFK;
-                                int col = kc0 + (n < NFI * NFK ? n % NFK : (n - NFI * NFK) % NFK);
-                                atomic_add(K + (size_t)row * nao + col, v);
-                            }
-                            kik[n] = 0.;
-                        }
-                        wave_sync();
-                    }
-                    if (lds_k) cur_k = run_k;
-                    if (!lds_k) {
-                        for (int n = tl; n < NFI*NFK + NFJ*NFK + NFI*NFL + NFJ*NFL; n += TS) kik[n] = 0.;
-                        wave_sync();
-                    }
-                    if (active) {
-#pragma unroll
-                        for (int q = 0; q < RPL; ++q) {
-                            if (rows[q] < 0) continue;
-                            const int ir = rows[q] / NFJ, jr = rows[q] % NFJ;
-                            // K_ik += sum_l D_jl ; K_jk += sum_l D_il
-#pragma unroll
-                            for (int k = 0; k < NFK; ++k) {
-                                double s1 = 0., s2 = 0.;
-#pragma unroll
-                                for (int l = 0; l < NFL; ++l) {
-                                    s1 = fma(acc[q][k*NFL + l], D[(j0+jr)*nao + l0+l], s1);
-                                    s2 = fma(acc[q][k*NFL + l], D[(i0+ir)*nao + l0+l], s2);
-                                }
-                                atomicAdd(kik + ir * NFK + k, s1);
-                                atomicAdd(kjk + jr * NFK + k, s2);
-                            }
-                            // K_il += sum_k D_jk ; K_jl += sum_k D_ik
-#pragma unroll
-                            for (int l = 0; l < NFL; ++l) {
-                                double s1 = 0., s2 = 0.;
-#pragma unroll
-                                for (int k = 0; k < NFK; ++k) {
-                                    s1 = fma(acc[q][k*NFL + l], D[(j0+jr)*nao + k0+k], s1);
-                                    s2 = fma(acc[q][k*NFL + l], D[(i0+ir)*nao + k0+k], s2);
-                                }
-                                if (lds_k) {
-                                    atomicAdd(kacc + ir * a.l_nao + l0 + l - a.l_ao0, s1);
-                                    atomicAdd(kacc + (NFI + jr) * a.l_nao + l0 + l - a.l_ao0, s2);
-                                } else {
-                                    atomicAdd(kil + ir * NFL + l, s1);
-                                    atomicAdd(kjl + jr * NFL + l, s2);
-                                }
-                            }
-                        }
-                    }
-                    wave_sync();
-                    if (!lds_k && active) {
-                        for (int n = tl; n < NFI*NFK + NFJ*NFK + NFI*NFL + NFJ*NFL; n += TS) {
-                            double v = kik[n];
-                            int row, col;
-                            if (n < NFI * NFK) { row = i0 + n / NFK; col = k0 + n % NFK; }
-                            else if (n < NFI * NFK + NFJ * NFK) {
-                                int m = n - NFI * NFK; row = j0 + m / NFK; col = k0 + m % NFK;
-                            } else if (n < NFI * NFK + NFJ * NFK + NFI * NFL) {
-                                int m = n - NFI * NFK - NFJ * NFK; row = i0 + m / NFL; col = l0 + m % NFL;
-                            } else {
-                                int m = n - NFI * NFK - NFJ * NFK - NFI * NFL;
-                                row = j0 + m / NFL; col = l0 + m % NFL;
-                            }
-                            atomic_add(K + (size_t)row * nao + col, v);
-                        }
-                        wave_sync();
-                    }
-                }
-                if (a.vj) {
-                    double *J = a.vj + id * nao2;
-                    for (int n = tl; n < NFKL; n += TS) jkl[n] = 0.;
-                    wave_sync();
-                    if (active) {
-#pragma unroll
-                        for (int q = 0; q < RPL; ++q) {
-                            if (rows[q] < 0) continue;
-                            const int ir = rows[q] / NFJ, jr = rows[q] % NFJ;
-                            const double dij = D[(i0+ir)*nao + j0+jr];
-                            double s = 0.;
+                            for (int n = 0; n < SK; ++n) { sx[n] = gx[n]; sy[n] = gy[n]; sz[n] = gz[n]; }
 #pragma unroll
                             for (int k = 0; k < NFK; ++k)
 #pragma unroll
                             for (int l = 0; l < NFL; ++l) {
-                                s = fma(acc[q][k*NFL + l], D[(k0+k)*nao + l0+l], s);
-                                atomicAdd(jkl + k*NFL + l, acc[q][k*NFL + l] * dij);
+                                int kx = ck.x[k]*SJ + cl.x[l];
+                                int ky = ck.y[k]*SJ + cl.y[l];
+                                int kz = ck.z[k]*SJ + cl.z[l];
+                                acc[k*NFL + l] = fma(sx[kx] * sy[ky], sz[kz], acc[k*NFL + l]);
                             }
-                            if (a.n_dm == 1) vjij[q] += s;
-                            else atomic_add(J + (i0+ir)*nao + j0+jr, s);
                         }
                     }
-                    wave_sync();
-                    if (active) {
-                        for (int n = tl; n < NFKL; n += TS)
-                            atomic_add(J + (k0 + n / NFL)*nao + l0 + n % NFL, jkl[n]);
-                    }
-                    wave_sync();
                 }
             }
-            // drop the NQW processed entries
-            const int nb = qn < NQW ? qn : NQW;
-            uint32_t rest = (nb + lane < qn) ? queue[nb + lane] : 0u;
-            wave_sync();
-            if (nb + lane < qn) queue[lane] = rest;
-            qn -= nb;
-            wave_sync();
-        }
-        if (flush && qn == 0) break;
-    }
-    // flush the open K_ik / K_jk run and the J_ij rows of this team
-    if (a.vk && kacc && cur_k >= 0) {
-        const int kc0 = a.ao_loc[cur_k];
-        for (int n = tl; n < NFI * NFK + NFJ * NFK; n += TS) {
-            double v = kik[n];
-            if (v != 0.) {
-                int row = n < NFI * NFK ? i0 + n / NFK : j0 + (n - NFI * NFK) / NFK;
-                int col = kc0 + (n < NFI * NFK ? n % NFK : (n - NFI * NFK) % NFK);
-                atomic_add(a.vk + (size_t)row * nao + col, v);
-            }
-        }
-    }
-    if (a.vj && a.n_dm == 1) {
+            // ---- digestion of the team's quartet
+            const int i0 = sp.i0, j0 = sp.j0;
+            const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
+            double *stage = my;       // reuse the team's LDS
+            for (int id = 0; id < a.n_dm; ++id) {
+                const double *D = a.dm + id * nao2;
+                if (a.vj && active && has_row) {
+                    double s = 0.;
 #pragma unroll
-        for (int q = 0; q < RPL; ++q)
-            if (rows[q] >= 0 && vjij[q] != 0.)
-                atomic_add(a.vj + (i0 + rows[q] / NFJ)*nao + j0 + rows[q] % NFJ, vjij[q]);
-    }
-    if (kacc) {
-        __syncthreads();
-        for (int n = threadIdx.x; n < nacc; n += WB * WAVE) {
-            double v = kacc[n];
-            if (v != 0.) {
-                int r = n / a.l_nao, c = n - r * a.l_nao;
-                int row = r < NFI ? i0 + r : j0 + r - NFI;
-                atomic_add(a.vk + (size_t)row * nao + a.l_ao0 + c, v);
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l)
+                        s = fma(acc[k*NFL + l], D[(k0+k)*nao + l0+l], s);
+                    atomic_add(a.vj + id * nao2 + (i0+irow)*nao + j0+jrow, s);
+                }
+                for (int r0 = 0; r0 < NROWS; r0 += S::S_ROWS) {
+                    const int r1 = min(r0 + S::S_ROWS, NROWS);
+                    __syncthreads();
+                    if (active && has_row && row >= r0 && row < r1) {
+#pragma unroll
+                        for (int n = 0; n < NFKL; ++n) stage[(row - r0) * NFKL + n] = acc[n];
+                    }
+                    __syncthreads();
+                    if (!active) continue;
+                    if (a.vj) {
+                        double *J = a.vj + id * nao2;
+                        for (int e = tl; e < NFKL; e += TS) {
+                            int k = e / NFL, l = e - (e / NFL) * NFL;
+                            double s = 0.;
+                            for (int rr = r0; rr < r1; ++rr)
+                                s = fma(stage[(rr - r0) * NFKL + e],
+                                        D[(i0 + rr / NFJ)*nao + j0 + rr % NFJ], s);
+                            atomic_add(J + (k0+k)*nao + l0+l, s);
+                        }
+                    }
+                    if (a.vk) {
+                        double *K = a.vk + id * nao2;
+                        const int ilo = r0 / NFJ, ihi = (r1 - 1) / NFJ;
+                        const int ni = ihi - ilo + 1;
+                        // K_ik (sum over j,l of D_jl) and K_il (sum over j,k of D_jk)
+                        for (int e = tl; e < ni * (NFK + NFL); e += TS) {
+                            int i = ilo + e / (NFK + NFL);
+                            int m = e - (e / (NFK + NFL)) * (NFK + NFL);
+                            double s = 0.;
+                            for (int j = 0; j < NFJ; ++j) {
+                                int rr = i * NFJ + j;
+                                if (rr < r0 || rr >= r1) continue;
+                                const double *srow = stage + (rr - r0) * NFKL;
+                                if (m < NFK) {
+                                    for (int l = 0; l < NFL; ++l)
+                                        s = fma(srow[m*NFL + l], D[(j0+j)*nao + l0+l], s);
+                                } else {
+                                    int l = m - NFK;
+                                    for (int k = 0; k < NFK; ++k)
+                                        s = fma(srow[k*NFL + l], D[(j0+j)*nao + k0+k], s);
+                                }
+                            }
+                            if (m < NFK) atomic_add(K + (i0+i)*nao + k0+m, s);
+                            else atomic_add(K + (i0+i)*nao + l0+m-NFK, s);
+                        }
+                        // K_jk (sum over i,l of D_il) and K_jl (sum over i,k of D_ik)
+                        for (int e = tl; e < NFJ * (NFK + NFL); e += TS) {
+                            int j = e / (NFK + NFL);
+                            int m = e - j * (NFK + NFL);
+                            double s = 0.;
+                            for (int i = ilo; i <= ihi; ++i) {
+                                int rr = i * NFJ + j;
+                                if (rr < r0 || rr >= r1) continue;
+                                const double *srow = stage + (rr - r0) * NFKL;
+                                if (m < NFK) {
+                                    for (int l = 0; l < NFL; ++l)
+                                        s = fma(srow[m*NFL + l], D[(i0+i)*nao + l0+l], s);
+                                } else {
+                                    int l = m - NFK;
+                                    for (int k = 0; k < NFK; ++k)
+                                        s = fma(srow[k*NFL + l], D[(i0+i)*nao + k0+k], s);
+                                }
+                            }
+                            if (m < NFK) atomic_add(K + (j0+j)*nao + k0+m, s);
+                            else atomic_add(K + (j0+j)*nao + l0+m-NFK, s);
+                        }
+                    }
+                }
             }
+            __syncthreads();
+            head += NQ;
         }
+        // move the unprocessed remainder to the front of the queue
+        int rem = qn - head;
+        uint32_t v = 0;
+        if (tid < rem) v = queue[head + tid];
+        __syncthreads();
+        if (tid < rem) queue[tid] = v;
+        qn = rem;
+        __syncthreads();
+        if (flush) break;
     }
-    if (a.n_quartets && lane == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
+    if (a.n_quartets && tid == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
 }
 
 // ---------------------------------------------------------------- dispatch
-// LDS bytes a workgroup may use on the current device (static + dynamic).
+// LDS bytes a workgroup may use (static + dynamic): the device limit, capped
+// at 64 KB.  gfx950 reports 160 KB per workgroup, but every launch of this
+// library stays within the 64 KB that has been validated on the hardware.
+#define JK_LDS_CAP (64 * 1024)
 static size_t lds_limit()
 {
     static int cached_dev = -1;
     static size_t cached = 0;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    if (hipGetDevice(&dev) != hipSuccess) return JK_LDS_CAP;
     if (dev != cached_dev) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess
-            || v <= 0)
-            v = 64 * 1024;
+            || v <= 0 || v > JK_LDS_CAP)
+            v = JK_LDS_CAP;
         cached = (size_t)v;
         cached_dev = dev;
     }
@@ -1099,21 +1023,12 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
                            use_lds ? lds : 0, stream, a, kl_chunk, use_lds);
     } else {
         using S = TeamShape<LI, LJ, LK, LL>;
-        int nchunks = (8192 + a.npairs_ij - 1) / a.npairs_ij;
-        int maxchunks = (a.npairs_kl + 127) / 128;
-        if (nchunks > maxchunks) nchunks = maxchunks;
-        if (nchunks < 1) nchunks = 1;
-        const int kl_chunk = (a.npairs_kl + nchunks - 1) / nchunks;
-        nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
+        const int kl_chunk = 256;
+        int nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-        constexpr int NFIJ = ncart(LI) + ncart(LJ);
-        size_t lds = (size_t)NFIJ * a.l_nao * sizeof(double);
-        size_t lds_static = sizeof(double) * S::WB * S::PER_WAVE +
-                            sizeof(uint32_t) * S::WB * 2 * WAVE;
-        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds + lds_static <= lds_limit();
-        hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB),
-                           use_lds ? lds : 0, stream, a, kl_chunk, use_lds);
+        hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB), 0,
+                           stream, a, kl_chunk);
     }
     return hipGetLastError();
 }
