@@ -101,6 +101,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-scf', action='store_true')
     ap.add_argument('--fp32-tol', type=float, default=1e-6)
+    ap.add_argument('--copy-ref', action='store_true',
+                    help='also run a 1 GiB device copy (reference for PMC byte counters)')
     ap.add_argument('--guess-model', default=None,
                     help='learned-guess weights (.npz) for the learned+mixed SCF')
     args = ap.parse_args()
@@ -138,50 +140,82 @@ def main():
         step()
     torch.cuda.synchronize()
     log('[bench] warmup done')
+    # timed region: every class launch is bracketed by HIP events on the
+    # stream it runs on, and its surviving quartets are counted on the device
+    opt.profile = True
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    prof = opt.profile_results()
+    opt.profile = False
     log(f'[bench] get_jk {dt*1e3:.1f} ms/step')
-
-    # per-class counts and kernel times (separate, untimed pass)
     dms_c = opt.apply_coeff_C_mat_CT(ddm[None])
-    opt.get_jk(dms_c, 1, count_quartets=True, time_classes=True)
-    counts = opt.last_task_counts
-    nq = opt.last_quartets
+
     uniq = opt.sorted_mol.uniq_l_ctr
-    flops = {}
-    for t, c in counts.items():
+    nq = sum(v[1] for v in prof.values()) // args.steps
+    kern_ms = sum(v[0] for v in prof.values()) / args.steps
+    total_flops = 0.
+    by_kernel = {}
+    for t, (ms, cnt, nl) in prof.items():
         ls = tuple(int(uniq[x, 0]) for x in t)
         ps = tuple(int(uniq[x, 1]) for x in t)
-        flops[t] = c * quartet_flops(ls, ps, 1)
-    ms = dict(opt.last_task_ms)
-    kern_ms = sum(ms.values())
-    total_flops = sum(flops.values())
-    # dominant angular class by kernel time
-    by_class = {}
-    for t, v in ms.items():
-        key = tuple(int(uniq[x, 0]) for x in t)
-        a = by_class.setdefault(key, [0., 0., 0])
-        a[0] += v; a[1] += flops[t]; a[2] += 1
-    dom = max(by_class, key=lambda k: by_class[k][0])
-    dom_ms, dom_flops, dom_n = by_class[dom]
+        f = cnt * quartet_flops(ls, ps, 1)
+        total_flops += f
+        a = by_kernel.setdefault(jk.class_kernel_name(ls), [0., 0., 0, 0])
+        a[0] += ms; a[1] += f; a[2] += nl; a[3] += cnt
+    dom = max(by_kernel, key=lambda k: by_kernel[k][0])
+    dom_ms, dom_flops, dom_n, dom_q = by_kernel[dom]
     peak = 78.6
-    log(f'[bench] quartets={nq} kernels {kern_ms:.1f} ms, {total_flops/kern_ms/1e9:.2f} TFLOP/s '
-        f'model; dominant class {dom} {dom_ms:.1f} ms')
-    for key in sorted(by_class, key=lambda k: -by_class[k][0])[:12]:
-        v = by_class[key]
-        log(f'    class {key}: {v[0]:.2f} ms  {v[1]/max(v[0],1e-9)/1e9:.2f} TFLOP/s  launches {v[2]}')
+    tflops = lambda f, ms: f / max(ms, 1e-12) / 1e9
+    log(f'[bench] quartets={nq} kernels {kern_ms:.1f} ms/step, '
+        f'{tflops(total_flops, kern_ms * args.steps):.2f} TFLOP/s model; '
+        f'dominant kernel {dom}: {dom_ms / args.steps:.1f} ms/step')
+    for key in sorted(by_kernel, key=lambda k: -by_kernel[k][0])[:16]:
+        v = by_kernel[key]
+        log(f'    {key}: {v[0] / args.steps:8.2f} ms/step  {tflops(v[1], v[0]):.2f} TFLOP/s  '
+            f'avg launch {v[0] / v[2] * 1e3:.1f} us  launches/step {v[2] // args.steps}')
+    # the same incremental build with FP32 quartets below fp32_tol
+    vj64, vk64 = opt.get_jk(dms_c, 1)
+    opt.get_jk(dms_c, 1, fp32_tol=args.fp32_tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vj32, vk32 = opt.get_jk(dms_c, 1, fp32_tol=args.fp32_tol, count_quartets=True)
+    torch.cuda.synchronize()
+    dt32 = (time.perf_counter() - t0) / args.steps
+    mixed = dict(fp32_tol=args.fp32_tol, ms_per_build=dt32 * 1e3,
+                 fp32_fraction=opt.last_quartets_fp32 / max(opt.last_quartets, 1),
+                 max_abs_err_j=float((vj32 - vj64).abs().max()),
+                 max_abs_err_k=float((vk32 - vk64).abs().max()))
+    log(f"[bench] mixed: {dt32*1e3:.1f} ms/step, fp32 fraction {mixed['fp32_fraction']:.3f}, "
+        f"max|dJ| {mixed['max_abs_err_j']:.2e} max|dK| {mixed['max_abs_err_k']:.2e}")
+    del vj64, vk64, vj32, vk32
+    if args.copy_ref:
+        # plain device copy of known size: the reference for PMC byte counters
+        nbytes = 1 << 30
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(); dst.copy_(src); e1.record()
+        torch.cuda.synchronize()
+        log(f'[bench] copy reference: {nbytes} B read + {nbytes} B written, '
+            f'{e0.elapsed_time(e1):.3f} ms ({2 * nbytes / e0.elapsed_time(e1) / 1e9:.2f} TB/s)')
+        del src, dst
     # compute bound: FP64 FMA, whose vector and matrix peaks coincide (78.6 TFLOP/s)
     roofline = dict(bound='mfma',
-                    kernel='jk class kernels, dominant class (%s)' % ''.join('spdf'[x] for x in dom),
-                    achieved=round(dom_flops / dom_ms / 1e9, 3),
+                    kernel=f'{dom} (all launches of the dominant kernel template)',
+                    achieved=round(tflops(dom_flops, dom_ms), 3),
                     peak=peak, unit='TFLOP/s',
-                    frac=round(dom_flops / dom_ms / 1e9 / peak, 4), traffic=None,
-                    all_classes=dict(achieved=round(total_flops / kern_ms / 1e9, 3),
-                                     frac=round(total_flops / kern_ms / 1e9 / peak, 4),
-                                     kernel_ms=round(kern_ms, 3)))
+                    frac=round(tflops(dom_flops, dom_ms) / peak, 4), traffic=None,
+                    avg_launch_us=round(dom_ms / dom_n * 1e3, 2),
+                    flops_per_launch=dom_flops / dom_n,
+                    quartets_per_launch=dom_q / dom_n,
+                    all_kernels=dict(achieved=round(tflops(total_flops, kern_ms * args.steps), 3),
+                                     frac=round(tflops(total_flops, kern_ms * args.steps) / peak, 4),
+                                     kernel_ms_per_step=round(kern_ms, 3)))
     out = dict(metric='J/K surviving shell quartets per second (one incremental get_jk, FP64)',
                value=nq / dt, unit='quartets/s', n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=dt * 1e3, higher_is_better=True,
@@ -189,7 +223,7 @@ def main():
                config=dict(workload=f'{args.config}: one get_jk(dD, hermi=1), '
                                     f'nao={mol.nao}, quartets={nq}',
                            global_batch=1, seq_len=0, parallelism='single GPU'),
-               roofline=roofline, setup_s=t_build)
+               roofline=roofline, setup_s=t_build, mixed=mixed)
     if not args.no_scf:
         out['scf'] = scf_leg(mol, args.fp32_tol, args.guess_model)
     if not args.no_cpu:

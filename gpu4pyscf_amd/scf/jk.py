@@ -75,6 +75,17 @@ def group_by_split_points(q, split_points):
     return np.concatenate(subsets).astype(np.int64)
 
 
+def class_kernel_name(ls):
+    '''Name of the HIP kernel template that evaluates angular class ls =
+    (li, lj, lk, ll): the per-lane kernel when the Cartesian block and the
+    three 2D tables fit in 120 doubles, else the team kernel.'''
+    nc = [(l + 1) * (l + 2) // 2 for l in ls]
+    nf = nc[0] * nc[1] * nc[2] * nc[3]
+    gs = (ls[0] + 1) * (ls[1] + 1) * (ls[2] + 1) * (ls[3] + 1)
+    kind = 'jk_lane_kernel' if nf + 3 * gs <= 120 else 'jk_team_kernel'
+    return f'{kind}<{ls[0]}, {ls[1]}, {ls[2]}, {ls[3]}>'
+
+
 class _VHFOpt:
     '''Screening object of the direct J/K build (gpu4pyscf/scf/jk.py:407-486).'''
 
@@ -88,6 +99,28 @@ class _VHFOpt:
         self._dev = None
         self.timings = {}
         self.last_quartets = None
+        # profiling mode: every get_jk records per-class HIP events and
+        # surviving-quartet counters without synchronising; profile_results()
+        # collects them
+        self.profile = False
+        self._profile_pending = []
+
+    def profile_results(self):
+        '''Per class task: (summed kernel milliseconds, surviving quartets,
+        launches) over the get_jk calls made since the last call, in
+        profiling mode.'''
+        torch = _torch()
+        torch.cuda.synchronize()
+        out = {}
+        for tasks, events, counter in self._profile_pending:
+            counts = counter.cpu().numpy().tolist()
+            for t, (e0, e1), c in zip(tasks, events, counts):
+                a = out.setdefault(t, [0., 0, 0])
+                a[0] += e0.elapsed_time(e1)
+                a[1] += int(c)
+                a[2] += 1
+        self._profile_pending = []
+        return out
 
     def reset(self, mol):
         self.mol = mol
@@ -250,6 +283,8 @@ class _VHFOpt:
         vj = torch.zeros_like(dms) if with_j else None
         vk = torch.zeros_like(dms) if with_k else None
         tasks = self.tasks()
+        prof = self.profile
+        count_quartets = count_quartets or prof
         counter = (torch.zeros(len(tasks), dtype=torch.int64, device=dms.device)
                    if count_quartets else None)
         counter32 = (torch.zeros(len(tasks), dtype=torch.int64, device=dms.device)
@@ -294,22 +329,24 @@ class _VHFOpt:
             args.q_kl = qkl.data_ptr()
             args.pdata_ij = d['pdata'][i, j].data_ptr()
             args.pdata_kl = d['pdata'][k, l].data_ptr()
-            if time_classes:
+            if time_classes or prof:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
             st = lib.jk_build(ctypes.byref(args), stream)
-            if time_classes:
+            if time_classes or prof:
                 e1.record()
                 events.append(((i, j, k, l), e0, e1))
             if st != 0:
                 clib.check(st, f'jk_build ({"spdfg"[args.li]}{"spdfg"[args.lj]}|'
                                f'{"spdfg"[args.lk]}{"spdfg"[args.ll]})')
-        if counter is not None:
+        if prof:
+            self._profile_pending.append((tasks, [(e0, e1) for _, e0, e1 in events], counter))
+        elif counter is not None:
             self.last_task_counts = dict(zip(tasks, counter.cpu().numpy().tolist()))
             self.last_quartets = int(sum(self.last_task_counts.values()))
             self.last_quartets_fp32 = int(counter32.sum().item())
-        if time_classes:
+        if time_classes and not prof:
             torch.cuda.synchronize()
             self.last_task_ms = [(t, e0.elapsed_time(e1)) for t, e0, e1 in events]
         if hermi == 1:
