@@ -1,15 +1,22 @@
 '''Train and evaluate the learned initial guess on seeded water clusters.
 
-python -m gpu4pyscf_amd.guess.train_water --train 0:200 --eval 5000:5020 \
-    --shape 2x2x2 --basis def2-svp --epochs 50 --out gpurun_out/mlp_water8_svp.npz
+python -m gpu4pyscf_amd.guess.train_water --train 1000:3000 --eval 5000:5200 \
+    --shape 2x2x2 --basis def2-svp --epochs 50 --workers 4 --out model.npz
 
-Prints one JSON line: training loss, and on the held-out seeds the SCF
-cycles/time to conv_tol with the learned guess against the atomic guess,
-plus the guess-density RMS error against the converged density.
+Training rows come from converged RHF densities of the seeded clusters,
+computed by the GPU SCF; with --workers N the seeds are split over N child
+processes (started before this process touches the GPU) that each write
+their rows to a scratch file.  Prints one JSON line: training loss, and on
+the held-out seeds the SCF cycles/time to conv_tol with the learned guess
+against the atomic guess, plus the guess-density RMS error against the
+converged density.
 '''
 import argparse
 import json
+import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -18,6 +25,38 @@ import numpy as np
 def _seeds(spec):
     a, b = spec.split(':')
     return list(range(int(a), int(b)))
+
+
+def _log(m):
+    print(m, file=sys.stderr, flush=True)
+
+
+def _generate(args, shape):
+    '''Training rows, in this process or split over child processes.'''
+    seeds = _seeds(args.train)
+    if args.workers <= 1:
+        from gpu4pyscf_amd.guess import learned as L
+        return L.make_dataset(seeds, shape, args.spacing, args.basis, log=_log)
+    tmp = tempfile.mkdtemp(prefix='guess_rows_')
+    parts = np.array_split(np.asarray(seeds), args.workers)
+    procs = []
+    for w, part in enumerate(parts):
+        if len(part) == 0:
+            continue
+        out = os.path.join(tmp, f'rows_{w}.npz')
+        cmd = [sys.executable, '-m', 'gpu4pyscf_amd.guess.train_water', '--gen-only', out,
+               '--train', f'{part[0]}:{part[-1] + 1}', '--shape', args.shape,
+               '--spacing', str(args.spacing), '--basis', args.basis]
+        procs.append((subprocess.Popen(cmd), out))
+    Xs, Ys, Ms = [], [], []
+    for p, out in procs:
+        if p.wait() != 0:
+            raise RuntimeError(f'data worker failed with status {p.returncode}')
+        d = np.load(out)
+        Xs.append(d['X']); Ys.append(d['Y']); Ms.append(d['M'])
+        os.remove(out)
+    os.rmdir(tmp)
+    return np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
 
 
 def main():
@@ -30,27 +69,36 @@ def main():
     ap.add_argument('--epochs', type=int, default=50)
     ap.add_argument('--lr', type=float, default=1e-3)
     ap.add_argument('--conv-tol', type=float, default=1e-8)
+    ap.add_argument('--workers', type=int, default=1)
     ap.add_argument('--out', default=None)
+    ap.add_argument('--gen-only', default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    shape = tuple(int(x) for x in args.shape.split('x'))
+    if args.gen_only:
+        from gpu4pyscf_amd.guess import learned as L
+        X, Y, M = L.make_dataset(_seeds(args.train), shape, args.spacing, args.basis, log=_log)
+        np.savez(args.gen_only, X=X, Y=Y, M=M)
+        return
+    t0 = time.perf_counter()
+    data = _generate(args, shape)       # before this process initialises the GPU
+    t_data = time.perf_counter() - t0
     import torch
     from gpu4pyscf_amd.guess import learned as L
     from gpu4pyscf_amd.scf.hf import RHF
-    log = lambda m: print(m, file=sys.stderr, flush=True)
-    shape = tuple(int(x) for x in args.shape.split('x'))
+    from gpu4pyscf_amd.scf.atom_guess import atom_guess
+    _log(f'[train_water] {data[0].shape[0]} pair rows from {len(_seeds(args.train))} '
+         f'geometries in {t_data:.1f} s')
     t0 = time.perf_counter()
-    data = L.make_dataset(_seeds(args.train), shape, args.spacing, args.basis, log=log)
-    t_data = time.perf_counter() - t0
-    log(f'[train_water] {data[0].shape[0]} pair rows from {len(_seeds(args.train))} '
-        f'geometries in {t_data:.1f} s')
-    t0 = time.perf_counter()
-    net, hist = L.train(data, epochs=args.epochs, lr=args.lr, log=log)
+    net, hist = L.train(data, epochs=args.epochs, lr=args.lr, log=_log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     guess = L.LearnedGuess(net)
     if args.out:
         guess.save(args.out)
     res = dict(n_train=len(_seeds(args.train)), rows=int(data[0].shape[0]),
-               t_data_s=t_data, t_train_s=t_train, final_mse=hist[-1], first_mse=hist[0])
+               t_data_s=t_data, t_train_s=t_train, final_mse=hist[-1], first_mse=hist[0],
+               epochs=args.epochs, workers=args.workers)
+    del data
     rows = []
     for s in _seeds(args.eval):
         mol = L.water_mol(s, shape, args.spacing, args.basis)
@@ -59,27 +107,30 @@ def main():
             mf = RHF(mol)
             mf.conv_tol = args.conv_tol
             mf.init_guess = key
+            torch.cuda.synchronize()
             t = time.perf_counter()
             e = mf.kernel()
+            torch.cuda.synchronize()
             out[name] = dict(e=e, cycles=mf.cycles, t=time.perf_counter() - t,
                              converged=mf.converged, jk=mf.jk_builds)
             if name == 'atom':
                 dconv = mf.dm.cpu().numpy()
-        d0 = guess(mol)
-        S = None
-        out['rms_learned'] = float(np.sqrt(np.mean((d0 - dconv)**2)))
-        from gpu4pyscf_amd.scf.atom_guess import atom_guess
+        out['rms_learned'] = float(np.sqrt(np.mean((guess(mol) - dconv)**2)))
         out['rms_atom'] = float(np.sqrt(np.mean((atom_guess(mol) - dconv)**2)))
+        out['fallback'] = guess.used_fallback
         rows.append(out)
-        log(f"[eval {s}] cycles atom {out['atom']['cycles']} learned {out['learned']['cycles']}"
-            f" dE {out['learned']['e'] - out['atom']['e']:.2e}")
+        _log(f"[eval {s}] cycles atom {out['atom']['cycles']} learned {out['learned']['cycles']}"
+             f" dE {out['learned']['e'] - out['atom']['e']:.2e}")
     for k in ('atom', 'learned'):
         res[f'cycles_{k}'] = float(np.mean([r[k]['cycles'] for r in rows]))
+        res[f'jk_builds_{k}'] = float(np.mean([r[k]['jk'] for r in rows]))
         res[f'time_{k}_s'] = float(np.mean([r[k]['t'] for r in rows]))
         res[f'converged_{k}'] = all(r[k]['converged'] for r in rows)
+    res['n_eval'] = len(rows)
     res['max_abs_dE'] = float(max(abs(r['learned']['e'] - r['atom']['e']) for r in rows))
     res['rms_guess_learned'] = float(np.mean([r['rms_learned'] for r in rows]))
     res['rms_guess_atom'] = float(np.mean([r['rms_atom'] for r in rows]))
+    res['fallbacks'] = int(sum(r['fallback'] for r in rows))
     print(json.dumps(res), flush=True)
 
 
