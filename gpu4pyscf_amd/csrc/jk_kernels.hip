@@ -700,7 +700,7 @@ struct TeamShape {
 
 template <int LI, int LJ, int LK, int LL>
 __global__ __launch_bounds__((TeamShape<LI, LJ, LK, LL>::TB))
-void jk_team_kernel(JKBuildArgs a, int kl_chunk)
+void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
 {
     using S = TeamShape<LI, LJ, LK, LL>;
     constexpr int NFI = S::NFI, NFJ = S::NFJ, NFK = S::NFK, NFL = S::NFL;
@@ -738,6 +738,16 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
     const int row = tl;               // (i, j) row owned by this lane
     const bool has_row = team < NQ && row < NROWS;
     const int irow = has_row ? row / NFJ : 0, jrow = has_row ? row % NFJ : 0;
+    // J_ij of this lane's row, summed over the block's quartets (one density)
+    double vj_row = 0.;
+    // K_il / K_jl rows of this work item accumulated in LDS (one density)
+    extern __shared__ double kacc_dyn[];
+    double *kacc = use_lds ? kacc_dyn : nullptr;
+    const int nacc = (NFI + NFJ) * a.l_nao;
+    if (kacc) {
+        for (int n = tid; n < nacc; n += TB) kacc[n] = 0.;
+    }
+    __syncthreads();
     int qn = 0;
     unsigned long long nsurv = 0;
     bool stop = false;
@@ -813,11 +823,10 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
                     double theta = aij * akl * r_aa;
                     double fac = pr[5] * Kkl * sqrt(r_aa);
                     __syncthreads();
-                    if (active && tl == 0) {
-                        double rt[N], wt[N];
-                        rys_roots<N>(a.rys, theta * rr, rt, wt);
-#pragma unroll
-                        for (int r = 0; r < N; ++r) { rw[2*r] = rt[r]; rw[2*r+1] = wt[r]; }
+                    if (active && tl < N) {
+                        double rt, wt;
+                        rys_root_one<N>(a.rys, theta * rr, tl, rt, wt);
+                        rw[2*tl] = rt; rw[2*tl+1] = wt;
                     }
                     __syncthreads();
                     if (active && tl < 3 * N) {
@@ -885,7 +894,8 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
 #pragma unroll
                     for (int l = 0; l < NFL; ++l)
                         s = fma(acc[k*NFL + l], D[(k0+k)*nao + l0+l], s);
-                    atomic_add(a.vj + id * nao2 + (i0+irow)*nao + j0+jrow, s);
+                    if (a.n_dm == 1) vj_row += s;
+                    else atomic_add(a.vj + id * nao2 + (i0+irow)*nao + j0+jrow, s);
                 }
                 for (int r0 = 0; r0 < NROWS; r0 += S::S_ROWS) {
                     const int r1 = min(r0 + S::S_ROWS, NROWS);
@@ -930,6 +940,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
                                 }
                             }
                             if (m < NFK) atomic_add(K + (i0+i)*nao + k0+m, s);
+                            else if (kacc) atomicAdd(kacc + i * a.l_nao + l0 + m - NFK - a.l_ao0, s);
                             else atomic_add(K + (i0+i)*nao + l0+m-NFK, s);
                         }
                         // K_jk (sum over i,l of D_il) and K_jl (sum over i,k of D_ik)
@@ -951,6 +962,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
                                 }
                             }
                             if (m < NFK) atomic_add(K + (j0+j)*nao + k0+m, s);
+                            else if (kacc) atomicAdd(kacc + (NFI + j) * a.l_nao + l0 + m - NFK - a.l_ao0, s);
                             else atomic_add(K + (j0+j)*nao + l0+m-NFK, s);
                         }
                     }
@@ -968,6 +980,19 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk)
         qn = rem;
         __syncthreads();
         if (flush) break;
+    }
+    if (a.vj && a.n_dm == 1 && has_row && vj_row != 0.)
+        atomic_add(a.vj + (sp.i0 + irow)*nao + sp.j0 + jrow, vj_row);
+    if (kacc) {
+        __syncthreads();
+        for (int n = tid; n < nacc; n += TB) {
+            double v = kacc[n];
+            if (v != 0.) {
+                int r = n / a.l_nao, c = n - r * a.l_nao;
+                int krow = r < NFI ? sp.i0 + r : sp.j0 + r - NFI;
+                atomic_add(a.vk + (size_t)krow * nao + a.l_ao0 + c, v);
+            }
+        }
     }
     if (a.n_quartets && tid == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
 }
@@ -1027,8 +1052,14 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         int nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-        hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB), 0,
-                           stream, a, kl_chunk);
+        constexpr int NFIJ = ncart(LI) + ncart(LJ);
+        size_t lds = (size_t)NFIJ * a.l_nao * sizeof(double);
+        // static LDS of the kernel: queue, wave counts, stop flag, team buffers
+        size_t lds_static = sizeof(uint32_t) * 2 * S::TB + sizeof(int) * (S::TB / WAVE + 2) +
+                            sizeof(double) * S::NQ * S::PER_Q + 256;
+        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds + lds_static <= lds_limit();
+        hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB),
+                           use_lds ? lds : 0, stream, a, kl_chunk, use_lds);
     }
     return hipGetLastError();
 }
