@@ -306,6 +306,50 @@ __device__ __forceinline__ WorkItem work_item(int npairs_kl, int kl_chunk)
     return w;
 }
 
+// K rows of one work item accumulated in LDS (one density): rows i (NFI)
+// then j (NFJ); columns: the k group's AOs (when kcols) followed by the l
+// group's AOs.  Flushed to K with one atomic per non-zero element.
+struct KRows {
+    double *p;          // nullptr: K goes straight to HBM
+    int width;          // doubles per row
+    int loff;           // column of the l group's first AO
+    bool kcols;         // K_ik / K_jk accumulated here too
+};
+
+__device__ __forceinline__ KRows make_krows(double *buf, int use_lds, const JKBuildArgs &a)
+{
+    KRows kr;
+    kr.p = use_lds ? buf : nullptr;
+    kr.kcols = use_lds == 2;
+    kr.loff = kr.kcols ? a.k_nao : 0;
+    kr.width = kr.loff + a.l_nao;
+    return kr;
+}
+
+// zero (before the block's first quartet; a barrier must follow)
+__device__ __forceinline__ void krows_zero(const KRows &kr, int nrows, int tid, int nthreads)
+{
+    if (!kr.p) return;
+    for (int n = tid; n < nrows * kr.width; n += nthreads) kr.p[n] = 0.;
+}
+
+// flush (after a barrier that ends the block's accumulation)
+__device__ __forceinline__ void krows_flush(const KRows &kr, int nfi, int nfj, int i0, int j0,
+                                            const JKBuildArgs &a, int tid, int nthreads)
+{
+    if (!kr.p) return;
+    const int n_all = (nfi + nfj) * kr.width;
+    for (int n = tid; n < n_all; n += nthreads) {
+        double v = kr.p[n];
+        if (v != 0.) {
+            int r = n / kr.width, c = n - r * kr.width;
+            int row = r < nfi ? i0 + r : j0 + r - nfi;
+            int col = c < kr.loff ? a.k_ao0 + c : a.l_ao0 + c - kr.loff;
+            atomic_add(a.vk + (size_t)row * a.nao + col, v);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- per-lane
 // Primitive-pair record, see jk_pair_data(): (a, 1/a, Px, Py, Pz, K).
 #define PREC 6
@@ -427,7 +471,7 @@ __device__ __forceinline__ void eri_lane(T *gout, const double *pij, int nij,
 template <int LI, int LJ, int LK, int LL, typename T>
 __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair &sp,
                                            uint32_t bas_ij, int pij, const uint32_t *queue,
-                                           int nb, int lane, uint64_t lt_mask, double *kacc)
+                                           int nb, int lane, uint64_t lt_mask, const KRows &kr)
 {
     constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
     constexpr int NF = NFI * NFJ * NFK * NFL;
@@ -485,15 +529,23 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                         kik[k] = fma(g, D[(j0+j)*nao + l0+l], kik[k]);
                     }
                 }
+                if (kr.kcols) {
+                    if (active) {
 #pragma unroll
-                for (int k = 0; k < NFK; ++k) {
-                    double v = seg_sum(kik[k], run, lane);
-                    if (tail) atomic_add(K + (i0+i)*nao + k0+k, v);
+                        for (int k = 0; k < NFK; ++k)
+                            atomicAdd(kr.p + i * kr.width + k0 + k - a.k_ao0, kik[k]);
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k) {
+                        double v = seg_sum(kik[k], run, lane);
+                        if (tail) atomic_add(K + (i0+i)*nao + k0+k, v);
+                    }
                 }
                 if (active) {
 #pragma unroll
                     for (int l = 0; l < NFL; ++l) {
-                        if (kacc) atomicAdd(kacc + i * a.l_nao + l0 + l - a.l_ao0, kil[l]);
+                        if (kr.p) atomicAdd(kr.p + i * kr.width + kr.loff + l0 + l - a.l_ao0, kil[l]);
                         else atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
                     }
                 }
@@ -518,15 +570,23 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                         kjk[k] = fma(g, D[(i0+i)*nao + l0+l], kjk[k]);
                     }
                 }
+                if (kr.kcols) {
+                    if (active) {
 #pragma unroll
-                for (int k = 0; k < NFK; ++k) {
-                    double v = seg_sum(kjk[k], run, lane);
-                    if (tail) atomic_add(K + (j0+j)*nao + k0+k, v);
+                        for (int k = 0; k < NFK; ++k)
+                            atomicAdd(kr.p + (NFI + j) * kr.width + k0 + k - a.k_ao0, kjk[k]);
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k) {
+                        double v = seg_sum(kjk[k], run, lane);
+                        if (tail) atomic_add(K + (j0+j)*nao + k0+k, v);
+                    }
                 }
                 if (active) {
 #pragma unroll
                     for (int l = 0; l < NFL; ++l) {
-                        if (kacc) atomicAdd(kacc + (NFI + j) * a.l_nao + l0 + l - a.l_ao0, kjl[l]);
+                        if (kr.p) atomicAdd(kr.p + (NFI + j) * kr.width + kr.loff + l0 + l - a.l_ao0, kjl[l]);
                         else atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
                     }
                 }
@@ -598,12 +658,11 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     if (a.q_kl[wi.kl0] + a.dm_penalty + Q_COND_MARGIN < kl_cutoff) return;
     ShellPair sp;
     load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
-    // K_il / K_jl rows of this work item accumulated in LDS (one density)
+    // K rows of this work item accumulated in LDS (one density)
     extern __shared__ double kacc_dyn[];
-    double *kacc = use_lds ? kacc_dyn : nullptr;
-    const int nacc = (NFI + NFJ) * a.l_nao;
-    if (kacc) {
-        for (int n = threadIdx.x; n < nacc; n += LANE_WAVES * WAVE) kacc[n] = 0.;
+    const KRows kr = make_krows(kacc_dyn, use_lds, a);
+    if (kr.p) {
+        krows_zero(kr, NFI + NFJ, threadIdx.x, LANE_WAVES * WAVE);
         __syncthreads();
     }
     int qn64 = 0, qn32 = 0;
@@ -638,7 +697,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
         if (qn64 >= WAVE || (flush && qn64 > 0)) {
             const int nb = min(qn64, WAVE);
             lane_batch<LI, LJ, LK, LL, double>(a, sp, bas_ij, wi.pij, queue64, nb, lane,
-                                               lt_mask, kacc);
+                                               lt_mask, kr);
             wave_sync();
             uint32_t rest = (nb + lane < qn64) ? queue64[nb + lane] : 0u;
             wave_sync();
@@ -649,7 +708,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
         if (qn32 >= WAVE || (flush && qn32 > 0)) {
             const int nb = min(qn32, WAVE);
             lane_batch<LI, LJ, LK, LL, float>(a, sp, bas_ij, wi.pij, queue32, nb, lane,
-                                              lt_mask, kacc);
+                                              lt_mask, kr);
             wave_sync();
             uint32_t rest = (nb + lane < qn32) ? queue32[nb + lane] : 0u;
             wave_sync();
@@ -659,16 +718,9 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
         }
         if (flush && qn64 == 0 && qn32 == 0) break;
     }
-    if (kacc) {
+    if (kr.p) {
         __syncthreads();
-        for (int n = threadIdx.x; n < nacc; n += LANE_WAVES * WAVE) {
-            double v = kacc[n];
-            if (v != 0.) {
-                int r = n / a.l_nao, c = n - r * a.l_nao;
-                int row = r < NFI ? sp.i0 + r : sp.j0 + r - NFI;
-                atomic_add(a.vk + (size_t)row * nao + a.l_ao0 + c, v);
-            }
-        }
+        krows_flush(kr, NFI, NFJ, sp.i0, sp.j0, a, threadIdx.x, LANE_WAVES * WAVE);
     }
     if (lane == 0) {
         if (a.n_quartets && nsurv) atomicAdd(a.n_quartets, nsurv);
@@ -740,13 +792,10 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     const int irow = has_row ? row / NFJ : 0, jrow = has_row ? row % NFJ : 0;
     // J_ij of this lane's row, summed over the block's quartets (one density)
     double vj_row = 0.;
-    // K_il / K_jl rows of this work item accumulated in LDS (one density)
+    // K rows of this work item accumulated in LDS (one density)
     extern __shared__ double kacc_dyn[];
-    double *kacc = use_lds ? kacc_dyn : nullptr;
-    const int nacc = (NFI + NFJ) * a.l_nao;
-    if (kacc) {
-        for (int n = tid; n < nacc; n += TB) kacc[n] = 0.;
-    }
+    const KRows kr = make_krows(kacc_dyn, use_lds, a);
+    krows_zero(kr, NFI + NFJ, tid, TB);
     __syncthreads();
     int qn = 0;
     unsigned long long nsurv = 0;
@@ -939,9 +988,14 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                                         s = fma(srow[k*NFL + l], D[(j0+j)*nao + k0+k], s);
                                 }
                             }
-                            if (m < NFK) atomic_add(K + (i0+i)*nao + k0+m, s);
-                            else if (kacc) atomicAdd(kacc + i * a.l_nao + l0 + m - NFK - a.l_ao0, s);
-                            else atomic_add(K + (i0+i)*nao + l0+m-NFK, s);
+                            if (m < NFK) {
+                                if (kr.kcols) atomicAdd(kr.p + i * kr.width + k0 + m - a.k_ao0, s);
+                                else atomic_add(K + (i0+i)*nao + k0+m, s);
+                            } else if (kr.p) {
+                                atomicAdd(kr.p + i * kr.width + kr.loff + l0 + m - NFK - a.l_ao0, s);
+                            } else {
+                                atomic_add(K + (i0+i)*nao + l0+m-NFK, s);
+                            }
                         }
                         // K_jk (sum over i,l of D_il) and K_jl (sum over i,k of D_ik)
                         for (int e = tl; e < NFJ * (NFK + NFL); e += TS) {
@@ -961,9 +1015,14 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                                         s = fma(srow[k*NFL + l], D[(i0+i)*nao + k0+k], s);
                                 }
                             }
-                            if (m < NFK) atomic_add(K + (j0+j)*nao + k0+m, s);
-                            else if (kacc) atomicAdd(kacc + (NFI + j) * a.l_nao + l0 + m - NFK - a.l_ao0, s);
-                            else atomic_add(K + (j0+j)*nao + l0+m-NFK, s);
+                            if (m < NFK) {
+                                if (kr.kcols) atomicAdd(kr.p + (NFI + j) * kr.width + k0 + m - a.k_ao0, s);
+                                else atomic_add(K + (j0+j)*nao + k0+m, s);
+                            } else if (kr.p) {
+                                atomicAdd(kr.p + (NFI + j) * kr.width + kr.loff + l0 + m - NFK - a.l_ao0, s);
+                            } else {
+                                atomic_add(K + (j0+j)*nao + l0+m-NFK, s);
+                            }
                         }
                     }
                 }
@@ -983,16 +1042,9 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     }
     if (a.vj && a.n_dm == 1 && has_row && vj_row != 0.)
         atomic_add(a.vj + (sp.i0 + irow)*nao + sp.j0 + jrow, vj_row);
-    if (kacc) {
+    if (kr.p) {
         __syncthreads();
-        for (int n = tid; n < nacc; n += TB) {
-            double v = kacc[n];
-            if (v != 0.) {
-                int r = n / a.l_nao, c = n - r * a.l_nao;
-                int krow = r < NFI ? sp.i0 + r : sp.j0 + r - NFI;
-                atomic_add(a.vk + (size_t)krow * nao + a.l_ao0 + c, v);
-            }
-        }
+        krows_flush(kr, NFI, NFJ, sp.i0, sp.j0, a, tid, TB);
     }
     if (a.n_quartets && tid == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
 }
@@ -1019,6 +1071,24 @@ static size_t lds_limit()
     return cached;
 }
 
+// LDS K rows for a launch: 2 = K_ik/K_jk and K_il/K_jl, 1 = K_il/K_jl only,
+// 0 = none (several densities, no K, or too little LDS); `bytes` receives
+// the dynamic LDS size.  max_mode 1 suits kernels whose many short work
+// items would spend more on zeroing and flushing the wider rows (and on the
+// occupancy they cost) than the K_ik/K_jk atomics they save.
+template <int LI, int LJ>
+static int krows_mode(const JKBuildArgs &a, size_t lds_static, int max_mode, size_t &bytes)
+{
+    constexpr size_t NFIJ = ncart(LI) + ncart(LJ);
+    bytes = 0;
+    if (!a.vk || a.n_dm != 1 || a.l_nao <= 0 || a.k_nao <= 0) return 0;
+    const size_t both = NFIJ * (size_t)(a.k_nao + a.l_nao) * sizeof(double);
+    const size_t lonly = NFIJ * (size_t)a.l_nao * sizeof(double);
+    if (max_mode >= 2 && both + lds_static <= lds_limit()) { bytes = both; return 2; }
+    if (lonly + lds_static <= lds_limit()) { bytes = lonly; return 1; }
+    return 0;
+}
+
 template <int LI, int LJ, int LK, int LL>
 struct ClassTraits {
     static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
@@ -1040,26 +1110,24 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-        constexpr int NFIJ = ncart(LI) + ncart(LJ);
-        size_t lds = (size_t)NFIJ * a.l_nao * sizeof(double);
-        size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * WAVE;
-        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds + lds_static <= lds_limit();
+        size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * 2 * WAVE;
+        size_t lds = 0;
+        const int use_lds = krows_mode<LI, LJ>(a, lds_static, 2, lds);
         hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(LANE_WAVES * WAVE),
-                           use_lds ? lds : 0, stream, a, kl_chunk, use_lds);
+                           lds, stream, a, kl_chunk, use_lds);
     } else {
         using S = TeamShape<LI, LJ, LK, LL>;
         const int kl_chunk = 256;
         int nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-        constexpr int NFIJ = ncart(LI) + ncart(LJ);
-        size_t lds = (size_t)NFIJ * a.l_nao * sizeof(double);
         // static LDS of the kernel: queue, wave counts, stop flag, team buffers
         size_t lds_static = sizeof(uint32_t) * 2 * S::TB + sizeof(int) * (S::TB / WAVE + 2) +
                             sizeof(double) * S::NQ * S::PER_Q + 256;
-        int use_lds = a.vk && a.n_dm == 1 && a.l_nao > 0 && lds + lds_static <= lds_limit();
+        size_t lds = 0;
+        const int use_lds = krows_mode<LI, LJ>(a, lds_static, 1, lds);
         hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB),
-                           use_lds ? lds : 0, stream, a, kl_chunk, use_lds);
+                           lds, stream, a, kl_chunk, use_lds);
     }
     return hipGetLastError();
 }
