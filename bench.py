@@ -59,34 +59,52 @@ def scf_leg(mol, fp32_tol, guess_model=None, conv_tol=1e-8):
     learned guess when a model is given); wall times include everything
     after the screening setup.'''
     import torch
+    from gpu4pyscf_amd.scf import jk
     from gpu4pyscf_amd.scf.hf import RHF
     res = {}
-    opt = None
+    opt = jk._VHFOpt(mol).build()      # screening setup shared by every variant
     variants = [('fp64', None, 'atom'), ('mixed', fp32_tol, 'atom')]
     if guess_model:
         from gpu4pyscf_amd.guess.learned import LearnedGuess
-        variants.append(('learned_mixed', fp32_tol, LearnedGuess.load(guess_model)))
+        lg = LearnedGuess.load(guess_model)
+        variants += [('learned', None, lg), ('learned_mixed', fp32_tol, lg)]
     for name, tol, guess in variants:
         mf = RHF(mol)
         mf.conv_tol = conv_tol
         mf.fp32_tol = tol
         mf.init_guess = guess
-        if opt is not None:
-            mf._opt = opt
+        mf._opt = opt
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         e = mf.kernel()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        opt = mf._opt
         res[name] = dict(e_tot=e, wall_s=dt, cycles=mf.cycles, converged=mf.converged,
                          jk_builds=mf.jk_builds, quartets=mf.quartets,
-                         fp32_fraction=mf.quartets_fp32 / max(mf.quartets, 1))
+                         fp32_fraction=mf.quartets_fp32 / max(mf.quartets, 1),
+                         history=[(float(t), float(x)) for t, x in zip(mf.history_t, mf.history)])
         log(f'[bench] SCF {name}: E={e:.10f} {dt:.2f} s, {mf.cycles} cycles, '
             f'fp32 {res[name]["fp32_fraction"]:.3f}')
     e0 = res['fp64']['e_tot']
+    for name in list(res):
+        r = res[name]
+        r['abs_dE_vs_fp64'] = abs(r['e_tot'] - e0)
+        # first SCF energy (after the initial build or a cycle) from which on
+        # every energy stays within 1e-6 Eh of the converged plain-FP64 energy
+        hist = r.pop('history')
+        t_hit = None
+        for n in range(len(hist)):
+            if all(abs(x - e0) < 1e-6 for _, x in hist[n:]):
+                t_hit = (n, hist[n][0])
+                break
+        r['cycles_to_1e-6'] = t_hit[0] if t_hit else None
+        r['time_to_1e-6_s'] = t_hit[1] if t_hit else None
+        log(f"[bench] SCF {name}: within 1e-6 Eh of plain FP64 after {r['cycles_to_1e-6']} "
+            f"cycles, {r['time_to_1e-6_s']} s")
+    t_ref = res['fp64']['time_to_1e-6_s']
     for name in res:
-        res[name]['abs_dE_vs_fp64'] = abs(res[name]['e_tot'] - e0)
+        t = res[name]['time_to_1e-6_s']
+        res[name]['speedup_to_1e-6_vs_fp64'] = (t_ref / t) if (t and t_ref) else None
     res['conv_tol'] = conv_tol
     res['fp32_tol'] = fp32_tol
     return res

@@ -42,18 +42,20 @@ def converged_density(mol, conv_tol=1e-9):
     return mf.dm.cpu().numpy(), mf
 
 
-def make_dataset(seeds, shape=(2, 2, 2), spacing=3.0, basis='def2-svp', log=None):
-    '''Rows (X, Y, mask) from converged densities of seeded clusters.'''
+def make_dataset(seeds, shape=(2, 2, 2), spacing=3.0, basis='def2-svp', log=None,
+                 label_tol=1e-9):
+    '''Rows (X, Y, mask) from densities of seeded clusters converged to
+    label_tol.'''
     Xs, Ys, Ms = [], [], []
     t0 = time.perf_counter()
     out_dim = F.max_block(water_mol(seeds[0], shape, spacing, basis))
     for n, s in enumerate(seeds):
         mol = water_mol(s, shape, spacing, basis)
-        dm, _ = converged_density(mol)
+        dm, _ = converged_density(mol, label_tol)
         rows, X = F.pairs_and_features(mol)
         Y, Mk = F.blocks_to_local(mol, dm, rows, out_dim)
         Xs.append(X); Ys.append(Y); Ms.append(Mk)
-        if log and (n + 1) % 25 == 0:
+        if log and ((n + 1) % 25 == 0 or n == 0):
             log(f'[dataset] {n+1}/{len(seeds)} geometries, {time.perf_counter()-t0:.1f} s')
     return (np.concatenate(Xs).astype(np.float32), np.concatenate(Ys).astype(np.float32),
             np.concatenate(Ms).astype(np.float32))
@@ -107,7 +109,8 @@ class LearnedGuess:
     def __call__(self, mol):
         from gpu4pyscf_amd.scf.atom_guess import atom_guess
         self.used_fallback = False
-        if self.net is None:
+        if self.net is None or F.max_block(mol) > self.net.dims[-1]:
+            # no network, or atom-pair blocks larger than its output
             self.used_fallback = True
             return atom_guess(mol)
         try:

@@ -36,7 +36,8 @@ def _generate(args, shape):
     seeds = _seeds(args.train)
     if args.workers <= 1:
         from gpu4pyscf_amd.guess import learned as L
-        return L.make_dataset(seeds, shape, args.spacing, args.basis, log=_log)
+        return L.make_dataset(seeds, shape, args.spacing, args.basis, log=_log,
+                              label_tol=args.label_tol)
     tmp = tempfile.mkdtemp(prefix='guess_rows_')
     parts = np.array_split(np.asarray(seeds), args.workers)
     procs = []
@@ -46,7 +47,8 @@ def _generate(args, shape):
         out = os.path.join(tmp, f'rows_{w}.npz')
         cmd = [sys.executable, '-m', 'gpu4pyscf_amd.guess.train_water', '--gen-only', out,
                '--train', f'{part[0]}:{part[-1] + 1}', '--shape', args.shape,
-               '--spacing', str(args.spacing), '--basis', args.basis]
+               '--spacing', str(args.spacing), '--basis', args.basis,
+               '--label-tol', str(args.label_tol)]
         procs.append((subprocess.Popen(cmd), out))
     Xs, Ys, Ms = [], [], []
     for p, out in procs:
@@ -69,6 +71,8 @@ def main():
     ap.add_argument('--epochs', type=int, default=50)
     ap.add_argument('--lr', type=float, default=1e-3)
     ap.add_argument('--conv-tol', type=float, default=1e-8)
+    ap.add_argument('--label-tol', type=float, default=1e-9,
+                    help='SCF convergence of the training densities')
     ap.add_argument('--workers', type=int, default=1)
     ap.add_argument('--out', default=None)
     ap.add_argument('--gen-only', default=None, help=argparse.SUPPRESS)
@@ -76,7 +80,8 @@ def main():
     shape = tuple(int(x) for x in args.shape.split('x'))
     if args.gen_only:
         from gpu4pyscf_amd.guess import learned as L
-        X, Y, M = L.make_dataset(_seeds(args.train), shape, args.spacing, args.basis, log=_log)
+        X, Y, M = L.make_dataset(_seeds(args.train), shape, args.spacing, args.basis, log=_log,
+                                 label_tol=args.label_tol)
         np.savez(args.gen_only, X=X, Y=Y, M=M)
         return
     t0 = time.perf_counter()

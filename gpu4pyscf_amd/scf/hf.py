@@ -117,7 +117,8 @@ class RHF:
         self.quartets_fp32 = 0
         self._fp32_now = None
         self.timings = {}
-        self.history = []
+        self.history = []           # energy after the initial build and each cycle
+        self.history_t = []         # seconds since kernel() started, per entry
         self.jk_hook = None
 
     # -- integrals -------------------------------------------------------
@@ -219,6 +220,7 @@ class RHF:
         diis = CDIIS(self.diis_space)
         self.converged = False
         self.history = [e_tot]
+        self.history_t = [time.perf_counter() - t_start]
         cycle = 0
         for cycle in range(self.max_cycle):
             dm_last = dm
@@ -240,6 +242,7 @@ class RHF:
             g = f_mo[~occ][:, occ]
             norm_gorb = float(torch.linalg.norm(g)) * 2.
             self.history.append(e_tot)
+            self.history_t.append(time.perf_counter() - t_start)
             if self.verbose:
                 print(f'cycle {cycle+1}: E = {e_tot:.12f} dE = {e_tot-last_e:.3e} '
                       f'|g| = {norm_gorb:.3e}', flush=True)
