@@ -55,7 +55,7 @@ def make_dataset(seeds, shape=(2, 2, 2), spacing=3.0, basis='def2-svp', log=None
         rows, X = F.pairs_and_features(mol)
         Y, Mk = F.blocks_to_local(mol, dm, rows, out_dim)
         Xs.append(X); Ys.append(Y); Ms.append(Mk)
-        if log and ((n + 1) % 25 == 0 or n == 0):
+        if log:
             log(f'[dataset] {n+1}/{len(seeds)} geometries, {time.perf_counter()-t0:.1f} s')
     return (np.concatenate(Xs).astype(np.float32), np.concatenate(Ys).astype(np.float32),
             np.concatenate(Ms).astype(np.float32))
