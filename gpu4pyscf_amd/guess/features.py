@@ -60,47 +60,87 @@ def _frame(z, x_hint):
     return np.stack([x, y, z])          # rows: local axes in lab coordinates
 
 
-def _sh_rotation(l, Q, _cache={}):
+def _sh_basis(l, _cache={}):
+    '''Sample points on the unit sphere, the monomial -> real-harmonic map
+    of angular momentum l, and the least-squares projector onto the
+    harmonics at those points.'''
+    if l not in _cache:
+        rng = np.random.default_rng(7)
+        pts = rng.standard_normal((64, 3))
+        pts /= np.linalg.norm(pts, axis=1)[:, None]
+        comps = np.asarray(cart_components(l))
+        C = c2s(l)
+        Y0 = (np.prod(pts[:, None, :] ** comps[None], axis=2)) @ C.T
+        _cache[l] = (pts, comps, C, np.linalg.pinv(Y0))
+    return _cache[l]
+
+
+def _sh_rotation(l, Q):
     '''U with Y_m(Q^T r) = sum_m' Y_m'(r) U[m', m] for PySCF-ordered real
     harmonics of angular momentum l (p ordered x, y, z).'''
     if l == 0:
         return np.ones((1, 1))
     if l == 1:
         return Q.copy()
-    key = l
-    if key not in _cache:
-        rng = np.random.default_rng(7)
-        pts = rng.standard_normal((64, 3))
-        _cache[key] = pts / np.linalg.norm(pts, axis=1)[:, None]
-    pts = _cache[key]
-    comps = cart_components(l)
-    C = c2s(l)
+    pts, comps, C, P = _sh_basis(l)
+    q = pts @ Q            # rows: (Q^T r)^T = r^T Q
+    Y1 = (np.prod(q[:, None, :] ** comps[None], axis=2)) @ C.T
+    return P @ Y1
 
-    def Y(p):
-        mono = np.stack([p[:, 0]**a * p[:, 1]**b * p[:, 2]**c for a, b, c in comps], axis=1)
-        return mono @ C.T
-    Y0 = Y(pts)
-    Y1 = Y(pts @ Q)        # rows: (Q^T r)^T = r^T Q
-    U, *_ = np.linalg.lstsq(Y0, Y1, rcond=None)
-    return U
+
+def _sh_rotations(l, Qs):
+    '''Batched _sh_rotation over frames Qs (R, 3, 3) -> (R, 2l+1, 2l+1).'''
+    R = len(Qs)
+    if l == 0:
+        return np.ones((R, 1, 1))
+    if l == 1:
+        return np.array(Qs, dtype=float)
+    pts, comps, C, P = _sh_basis(l)
+    q = pts @ Qs                                               # (R, 64, 3)
+    pw = [np.ones_like(q)]
+    for _ in range(l):
+        pw.append(pw[-1] * q)
+    pw = np.stack(pw, axis=-1)                                 # (R, 64, 3, l+1)
+    mono = (pw[:, :, 0, comps[:, 0]] * pw[:, :, 1, comps[:, 1]] * pw[:, :, 2, comps[:, 2]])
+    return P @ (mono @ C.T)
+
+
+def _shell_ls(mol, ia):
+    ls = []
+    for b in mol._bas[mol._bas[:, 0] == ia]:
+        ls += [int(b[ANG_OF])] * int(b[3])
+    return ls
+
+
+def ao_rotations(mol, atoms, Qs):
+    '''AO representations (spherical, PySCF order) of the frames Qs
+    (R, 3, 3) on the shells of ``atoms`` (R atoms of one element).'''
+    ls = _shell_ls(mol, atoms[0])
+    n = sum(2 * l + 1 for l in ls)
+    out = np.zeros((len(Qs), n, n))
+    rot = {l: _sh_rotations(l, Qs) for l in set(ls)}
+    p = 0
+    for l in ls:
+        k = 2 * l + 1
+        out[:, p:p + k, p:p + k] = rot[l]
+        p += k
+    return out
 
 
 def ao_rotation(mol, ia, Q):
     '''AO representation (spherical, PySCF order) of frame rotation Q on the
     shells of atom ia.'''
-    blocks = []
-    for b in mol._bas[mol._bas[:, 0] == ia]:
-        U = _sh_rotation(int(b[ANG_OF]), Q)
-        for _ in range(int(b[3])):
-            blocks.append(U)
-    n = sum(u.shape[0] for u in blocks)
-    out = np.zeros((n, n))
-    p = 0
-    for u in blocks:
-        k = u.shape[0]
-        out[p:p+k, p:p+k] = u
-        p += k
-    return out
+    return ao_rotations(mol, [ia], np.asarray(Q)[None])[0]
+
+
+def _groups(mol, rows):
+    '''Row indices grouped by (element of A, element of B): the rows of a
+    group share block shapes and shell structures.'''
+    sym = [mol.atom_symbol(i) for i in range(mol.natm)]
+    g = {}
+    for r, (a, b, t, Q) in enumerate(rows):
+        g.setdefault((sym[a], sym[b]), []).append(r)
+    return g
 
 
 def pairs_and_features(mol):
@@ -162,16 +202,19 @@ def blocks_to_local(mol, dm, rows, out_dim=OUT_DIM):
     aos = mol.aoslice_by_atom()
     Y = np.zeros((len(rows), out_dim))
     M = np.zeros((len(rows), out_dim))
-    for r, (a, b, t, Q) in enumerate(rows):
-        Ua = ao_rotation(mol, a, Q)
-        Ub = ao_rotation(mol, b, Q)
-        blk = dm[aos[a, 2]:aos[a, 3], aos[b, 2]:aos[b, 3]]
-        loc = Ua @ blk @ Ub.T
-        k = loc.size
-        if k > out_dim:
+    for idx in _groups(mol, rows).values():
+        A = [rows[r][0] for r in idx]
+        B = [rows[r][1] for r in idx]
+        Qs = np.stack([rows[r][3] for r in idx])
+        Ua = ao_rotations(mol, A, Qs)
+        Ub = ao_rotations(mol, B, Qs)
+        na, nb = Ua.shape[1], Ub.shape[1]
+        if na * nb > out_dim:
             raise ValueError('density block larger than the network output')
-        Y[r, :k] = loc.ravel()
-        M[r, :k] = 1.
+        blk = np.stack([dm[aos[a, 2]:aos[a, 3], aos[b, 2]:aos[b, 3]] for a, b in zip(A, B)])
+        loc = Ua @ blk @ Ub.transpose(0, 2, 1)
+        Y[idx, :na * nb] = loc.reshape(len(idx), -1)
+        M[idx, :na * nb] = 1.
     return Y, M
 
 
@@ -180,13 +223,21 @@ def local_to_density(mol, rows, Y):
     aos = mol.aoslice_by_atom()
     nao = mol.nao
     dm = np.zeros((nao, nao))
-    for r, (a, b, t, Q) in enumerate(rows):
-        Ua = ao_rotation(mol, a, Q)
-        Ub = ao_rotation(mol, b, Q)
-        na, nb = aos[a, 3] - aos[a, 2], aos[b, 3] - aos[b, 2]
-        loc = Y[r, :na * nb].reshape(na, nb)
-        blk = Ua.T @ loc @ Ub
-        dm[aos[a, 2]:aos[a, 3], aos[b, 2]:aos[b, 3]] = blk
-        if a != b:
-            dm[aos[b, 2]:aos[b, 3], aos[a, 2]:aos[a, 3]] = blk.T
+    for idx in _groups(mol, rows).values():
+        A = [rows[r][0] for r in idx]
+        B = [rows[r][1] for r in idx]
+        Qs = np.stack([rows[r][3] for r in idx])
+        Ua = ao_rotations(mol, A, Qs)
+        Ub = ao_rotations(mol, B, Qs)
+        na, nb = Ua.shape[1], Ub.shape[1]
+        if na * nb > Y.shape[1]:
+            raise ValueError('density block larger than the network output')
+        loc = np.asarray(Y)[idx, :na * nb].reshape(len(idx), na, nb)
+        blk = Ua.transpose(0, 2, 1) @ loc @ Ub
+        ia = aos[A, 2][:, None] + np.arange(na)                 # (R, na) AO rows
+        ib = aos[B, 2][:, None] + np.arange(nb)
+        # each (A, B) pair occurs once, so the scattered writes do not collide
+        dm[ia[:, :, None], ib[:, None, :]] = blk
+        off = np.asarray(A) != np.asarray(B)
+        dm[ib[off][:, :, None], ia[off][:, None, :]] = blk[off].transpose(0, 2, 1)
     return .5 * (dm + dm.T)

@@ -103,6 +103,9 @@ class RHF:
     # re-evaluated from one FP64 build of the full density (final_fp64)
     fp32_tol = None
     final_fp64 = True
+    # a callable (learned) guess is projected on the closest idempotent
+    # density with the right electron count before the first Fock build
+    project_guess = True
 
     def __init__(self, mol):
         self.mol = mol
@@ -180,6 +183,17 @@ class RHF:
     def make_rdm1(mo_coeff, mo_occ):
         return (mo_coeff * mo_occ) @ mo_coeff.T
 
+    def project_density(self, dm, S, X):
+        '''Closest closed-shell idempotent density: the nocc leading natural
+        orbitals of dm in the orthonormal basis X (X^T S X = 1), doubly
+        occupied.'''
+        torch = _torch()
+        d_orth = X.T @ S @ dm @ S @ X
+        w, v = torch.linalg.eigh(.5 * (d_orth + d_orth.T))
+        nocc = self.mol.nelectron // 2
+        c = X @ v[:, -nocc:]
+        return 2. * (c @ c.T)
+
     # -- initial guess ----------------------------------------------------
     def get_init_guess(self, h1e, S, X, key=None):
         torch = _torch()
@@ -191,6 +205,8 @@ class RHF:
             tr = float(torch.sum(dm * S))
             if tr > 0:
                 dm = dm * (self.mol.nelectron / tr)
+            if self.project_guess:
+                dm = self.project_density(dm, S, X)
             return dm
         if not isinstance(key, str):
             return torch.as_tensor(key, dtype=torch.float64, device=S.device)
