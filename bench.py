@@ -6,7 +6,11 @@ get_jk(dD, hermi=1) with dD = 1e-3 (G + G^T)/2, G ~ N(0,1) seeded, through
 the full drop-in boundary (AO transforms, dm_cond, all class launches,
 hermi post-processing).  Inputs are resident in HBM before timing.
 
-Prints one JSON line: value = surviving shell quartets per second.
+Prints one JSON line: value = surviving shell quartets per second.  The
+same line carries the roofline of the dominant kernel template, the
+mixed-precision build (error and FP32 fraction), the SCF leg (plain FP64,
+mixed, learned guess, learned + mixed: energies, cycles, time until the
+energy stays within 1e-6 Eh of the plain-FP64 one) and the CPU baseline.
 '''
 import argparse
 import json
@@ -121,8 +125,9 @@ def main():
     ap.add_argument('--fp32-tol', type=float, default=1e-6)
     ap.add_argument('--copy-ref', action='store_true',
                     help='also run a 1 GiB device copy (reference for PMC byte counters)')
-    ap.add_argument('--guess-model', default=None,
-                    help='learned-guess weights (.npz) for the learned+mixed SCF')
+    ap.add_argument('--guess-model', default='auto',
+                    help="learned-guess weights (.npz) for the learned SCF variants; 'auto': "
+                         "the shipped weights for the config's basis, 'none': no learned variants")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported',
@@ -243,7 +248,14 @@ def main():
                            global_batch=1, seq_len=0, parallelism='single GPU'),
                roofline=roofline, setup_s=t_build, mixed=mixed)
     if not args.no_scf:
-        out['scf'] = scf_leg(mol, args.fp32_tol, args.guess_model)
+        model = args.guess_model
+        if model == 'auto':
+            data = os.path.join(ROOT, 'gpu4pyscf_amd', 'guess', 'data')
+            model = os.path.join(data, {'def2-tzvp': 'water16_def2tzvp.npz',
+                                        'def2-svp': 'water8_def2svp.npz'}[basis])
+        elif model == 'none':
+            model = None
+        out['scf'] = scf_leg(mol, args.fp32_tol, model)
     if not args.no_cpu:
         log('[bench] CPU baseline (oracle) ...')
         out['cpu_baseline'] = cpu_baseline(opt, dms_c)
