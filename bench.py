@@ -58,6 +58,25 @@ def cpu_baseline(opt, dms_c, target_s=15.):
                        f'get_jk ({n} quartets, {t:.1f} s)')
 
 
+def measured_traffic(config, kernel):
+    '''HBM bytes per launch of ``kernel`` on ``config`` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries under profiles/ (None when no
+    summary covers that kernel and workload).'''
+    tag = config.replace('-', '_')
+    pdir = os.path.join(ROOT, 'profiles')
+    if not os.path.isdir(pdir):
+        return None, None
+    for fn in sorted(os.listdir(pdir)):
+        if tag not in fn or not fn.endswith('.md'):
+            continue
+        for line in open(os.path.join(pdir, fn)):
+            if line.startswith('traffic_bytes_per_launch '):
+                name, val = line[len('traffic_bytes_per_launch '):].rsplit('=', 1)
+                if name.strip() == kernel:
+                    return float(val), f'profiles/{fn}'
+    return None, None
+
+
 def scf_leg(mol, fp32_tol, guess_model=None, conv_tol=1e-8):
     '''Direct RHF to conv_tol: plain FP64 against mixed FP32/FP64 (and the
     learned guess when a model is given); wall times include everything
@@ -227,12 +246,15 @@ def main():
         log(f'[bench] copy reference: {nbytes} B read + {nbytes} B written, '
             f'{e0.elapsed_time(e1):.3f} ms ({2 * nbytes / e0.elapsed_time(e1) / 1e9:.2f} TB/s)')
         del src, dst
+    traffic, traffic_src = measured_traffic(args.config, dom)
     # compute bound: FP64 FMA, whose vector and matrix peaks coincide (78.6 TFLOP/s)
     roofline = dict(bound='mfma',
                     kernel=f'{dom} (all launches of the dominant kernel template)',
                     achieved=round(tflops(dom_flops, dom_ms), 3),
                     peak=peak, unit='TFLOP/s',
-                    frac=round(tflops(dom_flops, dom_ms) / peak, 4), traffic=None,
+                    frac=round(tflops(dom_flops, dom_ms) / peak, 4), traffic=traffic,
+                    traffic_unit='bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)',
+                    traffic_source=traffic_src,
                     avg_launch_us=round(dom_ms / dom_n * 1e3, 2),
                     flops_per_launch=dom_flops / dom_n,
                     quartets_per_launch=dom_q / dom_n,
