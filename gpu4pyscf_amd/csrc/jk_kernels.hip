@@ -748,6 +748,12 @@ struct TeamShape {
     static constexpr int NQ1 = (6144 / PER_Q) > 0 ? (6144 / PER_Q) : 1;   // 48 KB of LDS
     static constexpr int NQ = NQ0 < NQ1 ? NQ0 : NQ1;
     static constexpr int TS = TB / NQ;
+    // per-team staging of the density blocks of the current quartet:
+    // D_kl, then rows r of i (r < NFI) and j with their k and l columns,
+    // then D_ij (used when several densities are digested)
+    static constexpr int DROW = NFK + NFL;
+    static constexpr int D_KL = 0, D_R = NFKL, D_IJ = D_R + (NFI + NFJ) * DROW;
+    static constexpr int DSZ = D_IJ + NFI * NFJ;
 };
 
 template <int LI, int LJ, int LK, int LL>
@@ -769,6 +775,8 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     __shared__ int wave_cnt[NW + 1];
     __shared__ int stop_flag;
     __shared__ double lds[NQ * S::PER_Q];
+    __shared__ double dstage[NQ * S::DSZ];
+    __shared__ double dij_blk[NFI * NFJ];     // D_ij of the work item (one density)
 
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -796,6 +804,9 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     extern __shared__ double kacc_dyn[];
     const KRows kr = make_krows(kacc_dyn, use_lds, a);
     krows_zero(kr, NFI + NFJ, tid, TB);
+    for (int n = tid; n < NFI * NFJ; n += TB)
+        dij_blk[n] = a.dm[(size_t)(sp.i0 + n / NFJ) * nao + sp.j0 + n % NFJ];
+    double *dsh = dstage + (team < NQ ? team : 0) * S::DSZ;
     __syncthreads();
     int qn = 0;
     unsigned long long nsurv = 0;
@@ -934,15 +945,39 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
             const int i0 = sp.i0, j0 = sp.j0;
             const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
             double *stage = my;       // reuse the team's LDS
+            constexpr int DROW = S::DROW;
+            const double *dkl = dsh + S::D_KL, *drow = dsh + S::D_R;
             for (int id = 0; id < a.n_dm; ++id) {
                 const double *D = a.dm + id * nao2;
+                // the quartet's density blocks into the team's LDS, one
+                // cooperative gather
+                __syncthreads();
+                if (active) {
+                    for (int n = tl; n < S::DSZ; n += TS) {
+                        size_t idx;
+                        if (n < S::D_R) {
+                            idx = (size_t)(k0 + n / NFL) * nao + l0 + n % NFL;
+                        } else if (n < S::D_IJ) {
+                            const int r = (n - S::D_R) / DROW, c = (n - S::D_R) - r * DROW;
+                            const int ao = r < NFI ? i0 + r : j0 + r - NFI;
+                            idx = (size_t)ao * nao + (c < NFK ? k0 + c : l0 + c - NFK);
+                        } else {
+                            if (a.n_dm == 1) continue;
+                            const int m = n - S::D_IJ;
+                            idx = (size_t)(i0 + m / NFJ) * nao + j0 + m % NFJ;
+                        }
+                        dsh[n] = D[idx];
+                    }
+                }
+                __syncthreads();
+                const double *dij = a.n_dm == 1 ? dij_blk : dsh + S::D_IJ;
                 if (a.vj && active && has_row) {
                     double s = 0.;
 #pragma unroll
                     for (int k = 0; k < NFK; ++k)
 #pragma unroll
                     for (int l = 0; l < NFL; ++l)
-                        s = fma(acc[k*NFL + l], D[(k0+k)*nao + l0+l], s);
+                        s = fma(acc[k*NFL + l], dkl[k*NFL + l], s);
                     if (a.n_dm == 1) vj_row += s;
                     else atomic_add(a.vj + id * nao2 + (i0+irow)*nao + j0+jrow, s);
                 }
@@ -961,8 +996,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                             int k = e / NFL, l = e - (e / NFL) * NFL;
                             double s = 0.;
                             for (int rr = r0; rr < r1; ++rr)
-                                s = fma(stage[(rr - r0) * NFKL + e],
-                                        D[(i0 + rr / NFJ)*nao + j0 + rr % NFJ], s);
+                                s = fma(stage[(rr - r0) * NFKL + e], dij[rr], s);
                             atomic_add(J + (k0+k)*nao + l0+l, s);
                         }
                     }
@@ -979,13 +1013,14 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                                 int rr = i * NFJ + j;
                                 if (rr < r0 || rr >= r1) continue;
                                 const double *srow = stage + (rr - r0) * NFKL;
+                                const double *dj = drow + (NFI + j) * DROW;
                                 if (m < NFK) {
                                     for (int l = 0; l < NFL; ++l)
-                                        s = fma(srow[m*NFL + l], D[(j0+j)*nao + l0+l], s);
+                                        s = fma(srow[m*NFL + l], dj[NFK + l], s);
                                 } else {
                                     int l = m - NFK;
                                     for (int k = 0; k < NFK; ++k)
-                                        s = fma(srow[k*NFL + l], D[(j0+j)*nao + k0+k], s);
+                                        s = fma(srow[k*NFL + l], dj[k], s);
                                 }
                             }
                             if (m < NFK) {
@@ -1006,13 +1041,14 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                                 int rr = i * NFJ + j;
                                 if (rr < r0 || rr >= r1) continue;
                                 const double *srow = stage + (rr - r0) * NFKL;
+                                const double *di = drow + i * DROW;
                                 if (m < NFK) {
                                     for (int l = 0; l < NFL; ++l)
-                                        s = fma(srow[m*NFL + l], D[(i0+i)*nao + l0+l], s);
+                                        s = fma(srow[m*NFL + l], di[NFK + l], s);
                                 } else {
                                     int l = m - NFK;
                                     for (int k = 0; k < NFK; ++k)
-                                        s = fma(srow[k*NFL + l], D[(i0+i)*nao + k0+k], s);
+                                        s = fma(srow[k*NFL + l], di[k], s);
                                 }
                             }
                             if (m < NFK) {
@@ -1123,7 +1159,8 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
         // static LDS of the kernel: queue, wave counts, stop flag, team buffers
         size_t lds_static = sizeof(uint32_t) * 2 * S::TB + sizeof(int) * (S::TB / WAVE + 2) +
-                            sizeof(double) * S::NQ * S::PER_Q + 256;
+                            sizeof(double) * (S::NQ * (S::PER_Q + S::DSZ) + S::NFI * S::NFJ) +
+                            256;
         size_t lds = 0;
         const int use_lds = krows_mode<LI, LJ>(a, lds_static, 1, lds);
         hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB),
