@@ -61,6 +61,13 @@ __device__ __forceinline__ void atomic_add(double *p, double v)
     unsafeAtomicAdd(p, v);
 }
 
+// LDS accumulation: workgroup scope, so that it cannot be merged with the
+// device-scope HBM atomics of a sibling branch into one generic atomic
+__device__ __forceinline__ void lds_add(double *p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -309,27 +316,56 @@ __device__ __forceinline__ WorkItem work_item(int npairs_kl, int kl_chunk)
 // K rows of one work item accumulated in LDS (one density): rows i (NFI)
 // then j (NFJ); columns: the k group's AOs (when kcols) followed by the l
 // group's AOs.  Flushed to K with one atomic per non-zero element.
+// The same row layout also holds, after the K rows, a copy of the density
+// rows i and j over the k and l group columns (dstaged), from which the
+// per-lane digestion reads D_ik, D_il, D_jk and D_jl.
+// p and d always point into the LDS array (so that every access compiles
+// to an LDS instruction); `on` and `dstaged` say whether they are used.
 struct KRows {
-    double *p;          // nullptr: K goes straight to HBM
+    double *p;          // K rows
+    const double *d;    // density rows
     int width;          // doubles per row
     int loff;           // column of the l group's first AO
+    bool on;            // K_il / K_jl (and with kcols K_ik / K_jk) accumulated here
     bool kcols;         // K_ik / K_jk accumulated here too
+    bool dstaged;       // density rows present
 };
 
-__device__ __forceinline__ KRows make_krows(double *buf, int use_lds, const JKBuildArgs &a)
+// use_lds: 0 none, 1 K_il/K_jl rows, 2 K_ik/K_jk and K_il/K_jl rows,
+// 2|4 the same plus the density rows
+__device__ __forceinline__ KRows make_krows(double *buf, int use_lds, int nrows,
+                                            const JKBuildArgs &a)
 {
     KRows kr;
-    kr.p = use_lds ? buf : nullptr;
-    kr.kcols = use_lds == 2;
+    kr.on = (use_lds & 3) != 0;
+    kr.kcols = (use_lds & 3) == 2;
+    kr.dstaged = kr.kcols && (use_lds & 4);
     kr.loff = kr.kcols ? a.k_nao : 0;
     kr.width = kr.loff + a.l_nao;
+    kr.p = buf;
+    kr.d = buf + nrows * kr.width;
     return kr;
+}
+
+// copy the density rows (after krows_zero; a barrier must follow)
+__device__ __forceinline__ void krows_load_d(const KRows &kr, int nfi, int nfj, int i0, int j0,
+                                             const JKBuildArgs &a, int tid, int nthreads)
+{
+    if (!kr.dstaged) return;
+    double *d = kr.p + (nfi + nfj) * kr.width;
+    const int n_all = (nfi + nfj) * kr.width;
+    for (int n = tid; n < n_all; n += nthreads) {
+        int r = n / kr.width, c = n - r * kr.width;
+        int row = r < nfi ? i0 + r : j0 + r - nfi;
+        int col = c < kr.loff ? a.k_ao0 + c : a.l_ao0 + c - kr.loff;
+        d[n] = a.dm[(size_t)row * a.nao + col];
+    }
 }
 
 // zero (before the block's first quartet; a barrier must follow)
 __device__ __forceinline__ void krows_zero(const KRows &kr, int nrows, int tid, int nthreads)
 {
-    if (!kr.p) return;
+    if (!kr.on) return;
     for (int n = tid; n < nrows * kr.width; n += nthreads) kr.p[n] = 0.;
 }
 
@@ -337,7 +373,7 @@ __device__ __forceinline__ void krows_zero(const KRows &kr, int nrows, int tid, 
 __device__ __forceinline__ void krows_flush(const KRows &kr, int nfi, int nfj, int i0, int j0,
                                             const JKBuildArgs &a, int tid, int nthreads)
 {
-    if (!kr.p) return;
+    if (!kr.on) return;
     const int n_all = (nfi + nfj) * kr.width;
     for (int n = tid; n < n_all; n += nthreads) {
         double v = kr.p[n];
@@ -505,6 +541,8 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
     const int run = __popcll(heads & (lt_mask | (1ull << lane)));
     const int run_next = __shfl_down(run, 1, WAVE);
     const bool tail = active && (lane == nb - 1 || run_next != run);
+    // columns of this quartet's k and l AOs in the LDS density rows
+    const int dk = k0 - a.k_ao0, dl = kr.loff + l0 - a.l_ao0;
     for (int id = 0; id < a.n_dm; ++id) {
         const double *D = a.dm + id * nao2;
         if (a.vk) {
@@ -517,7 +555,18 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 for (int l = 0; l < NFL; ++l) kil[l] = 0.;
 #pragma unroll
                 for (int k = 0; k < NFK; ++k) kik[k] = 0.;
-                if (active) {
+                if (active && kr.dstaged) {
+#pragma unroll
+                    for (int j = 0; j < NFJ; ++j)
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) {
+                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
+                        kil[l] = fma(g, kr.d[(NFI + j) * kr.width + dk + k], kil[l]);
+                        kik[k] = fma(g, kr.d[(NFI + j) * kr.width + dl + l], kik[k]);
+                    }
+                } else if (active) {
 #pragma unroll
                     for (int j = 0; j < NFJ; ++j)
 #pragma unroll
@@ -533,7 +582,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                     if (active) {
 #pragma unroll
                         for (int k = 0; k < NFK; ++k)
-                            atomicAdd(kr.p + i * kr.width + k0 + k - a.k_ao0, kik[k]);
+                            lds_add(kr.p + i * kr.width + k0 + k - a.k_ao0, kik[k]);
                     }
                 } else {
 #pragma unroll
@@ -545,7 +594,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 if (active) {
 #pragma unroll
                     for (int l = 0; l < NFL; ++l) {
-                        if (kr.p) atomicAdd(kr.p + i * kr.width + kr.loff + l0 + l - a.l_ao0, kil[l]);
+                        if (kr.on) lds_add(kr.p + i * kr.width + kr.loff + l0 + l - a.l_ao0, kil[l]);
                         else atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
                     }
                 }
@@ -558,7 +607,18 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 for (int l = 0; l < NFL; ++l) kjl[l] = 0.;
 #pragma unroll
                 for (int k = 0; k < NFK; ++k) kjk[k] = 0.;
-                if (active) {
+                if (active && kr.dstaged) {
+#pragma unroll
+                    for (int i = 0; i < NFI; ++i)
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                    for (int l = 0; l < NFL; ++l) {
+                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
+                        kjl[l] = fma(g, kr.d[i * kr.width + dk + k], kjl[l]);
+                        kjk[k] = fma(g, kr.d[i * kr.width + dl + l], kjk[k]);
+                    }
+                } else if (active) {
 #pragma unroll
                     for (int i = 0; i < NFI; ++i)
 #pragma unroll
@@ -574,7 +634,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                     if (active) {
 #pragma unroll
                         for (int k = 0; k < NFK; ++k)
-                            atomicAdd(kr.p + (NFI + j) * kr.width + k0 + k - a.k_ao0, kjk[k]);
+                            lds_add(kr.p + (NFI + j) * kr.width + k0 + k - a.k_ao0, kjk[k]);
                     }
                 } else {
 #pragma unroll
@@ -586,7 +646,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 if (active) {
 #pragma unroll
                     for (int l = 0; l < NFL; ++l) {
-                        if (kr.p) atomicAdd(kr.p + (NFI + j) * kr.width + kr.loff + l0 + l - a.l_ao0, kjl[l]);
+                        if (kr.on) lds_add(kr.p + (NFI + j) * kr.width + kr.loff + l0 + l - a.l_ao0, kjl[l]);
                         else atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
                     }
                 }
@@ -660,9 +720,10 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
     // K rows of this work item accumulated in LDS (one density)
     extern __shared__ double kacc_dyn[];
-    const KRows kr = make_krows(kacc_dyn, use_lds, a);
-    if (kr.p) {
+    const KRows kr = make_krows(kacc_dyn, use_lds, NFI + NFJ, a);
+    if (kr.on) {
         krows_zero(kr, NFI + NFJ, threadIdx.x, LANE_WAVES * WAVE);
+        krows_load_d(kr, NFI, NFJ, sp.i0, sp.j0, a, threadIdx.x, LANE_WAVES * WAVE);
         __syncthreads();
     }
     int qn64 = 0, qn32 = 0;
@@ -718,7 +779,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
         }
         if (flush && qn64 == 0 && qn32 == 0) break;
     }
-    if (kr.p) {
+    if (kr.on) {
         __syncthreads();
         krows_flush(kr, NFI, NFJ, sp.i0, sp.j0, a, threadIdx.x, LANE_WAVES * WAVE);
     }
@@ -802,7 +863,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     double vj_row = 0.;
     // K rows of this work item accumulated in LDS (one density)
     extern __shared__ double kacc_dyn[];
-    const KRows kr = make_krows(kacc_dyn, use_lds, a);
+    const KRows kr = make_krows(kacc_dyn, use_lds, NFI + NFJ, a);
     krows_zero(kr, NFI + NFJ, tid, TB);
     for (int n = tid; n < NFI * NFJ; n += TB)
         dij_blk[n] = a.dm[(size_t)(sp.i0 + n / NFJ) * nao + sp.j0 + n % NFJ];
@@ -1024,10 +1085,10 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                                 }
                             }
                             if (m < NFK) {
-                                if (kr.kcols) atomicAdd(kr.p + i * kr.width + k0 + m - a.k_ao0, s);
+                                if (kr.kcols) lds_add(kr.p + i * kr.width + k0 + m - a.k_ao0, s);
                                 else atomic_add(K + (i0+i)*nao + k0+m, s);
-                            } else if (kr.p) {
-                                atomicAdd(kr.p + i * kr.width + kr.loff + l0 + m - NFK - a.l_ao0, s);
+                            } else if (kr.on) {
+                                lds_add(kr.p + i * kr.width + kr.loff + l0 + m - NFK - a.l_ao0, s);
                             } else {
                                 atomic_add(K + (i0+i)*nao + l0+m-NFK, s);
                             }
@@ -1052,10 +1113,10 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                                 }
                             }
                             if (m < NFK) {
-                                if (kr.kcols) atomicAdd(kr.p + (NFI + j) * kr.width + k0 + m - a.k_ao0, s);
+                                if (kr.kcols) lds_add(kr.p + (NFI + j) * kr.width + k0 + m - a.k_ao0, s);
                                 else atomic_add(K + (j0+j)*nao + k0+m, s);
-                            } else if (kr.p) {
-                                atomicAdd(kr.p + (NFI + j) * kr.width + kr.loff + l0 + m - NFK - a.l_ao0, s);
+                            } else if (kr.on) {
+                                lds_add(kr.p + (NFI + j) * kr.width + kr.loff + l0 + m - NFK - a.l_ao0, s);
                             } else {
                                 atomic_add(K + (j0+j)*nao + l0+m-NFK, s);
                             }
@@ -1078,7 +1139,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     }
     if (a.vj && a.n_dm == 1 && has_row && vj_row != 0.)
         atomic_add(a.vj + (sp.i0 + irow)*nao + sp.j0 + jrow, vj_row);
-    if (kr.p) {
+    if (kr.on) {
         __syncthreads();
         krows_flush(kr, NFI, NFJ, sp.i0, sp.j0, a, tid, TB);
     }
@@ -1113,13 +1174,18 @@ static size_t lds_limit()
 // items would spend more on zeroing and flushing the wider rows (and on the
 // occupancy they cost) than the K_ik/K_jk atomics they save.
 template <int LI, int LJ>
-static int krows_mode(const JKBuildArgs &a, size_t lds_static, int max_mode, size_t &bytes)
+static int krows_mode(const JKBuildArgs &a, size_t lds_static, int max_mode, bool drows,
+                      size_t &bytes)
 {
     constexpr size_t NFIJ = ncart(LI) + ncart(LJ);
     bytes = 0;
     if (!a.vk || a.n_dm != 1 || a.l_nao <= 0 || a.k_nao <= 0) return 0;
     const size_t both = NFIJ * (size_t)(a.k_nao + a.l_nao) * sizeof(double);
     const size_t lonly = NFIJ * (size_t)a.l_nao * sizeof(double);
+    if (max_mode >= 2 && drows && 2 * both + lds_static <= lds_limit()) {
+        bytes = 2 * both;
+        return 2 | 4;
+    }
     if (max_mode >= 2 && both + lds_static <= lds_limit()) { bytes = both; return 2; }
     if (lonly + lds_static <= lds_limit()) { bytes = lonly; return 1; }
     return 0;
@@ -1148,7 +1214,7 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
         size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * 2 * WAVE;
         size_t lds = 0;
-        const int use_lds = krows_mode<LI, LJ>(a, lds_static, 2, lds);
+        const int use_lds = krows_mode<LI, LJ>(a, lds_static, 2, true, lds);
         hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(LANE_WAVES * WAVE),
                            lds, stream, a, kl_chunk, use_lds);
     } else {
@@ -1162,7 +1228,7 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
                             sizeof(double) * (S::NQ * (S::PER_Q + S::DSZ) + S::NFI * S::NFJ) +
                             256;
         size_t lds = 0;
-        const int use_lds = krows_mode<LI, LJ>(a, lds_static, 1, lds);
+        const int use_lds = krows_mode<LI, LJ>(a, lds_static, 1, false, lds);
         hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB),
                            lds, stream, a, kl_chunk, use_lds);
     }
