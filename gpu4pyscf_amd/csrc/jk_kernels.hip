@@ -28,6 +28,21 @@
 #define WAVE            64
 #define RYS_DEG_MAX     15
 #define LANE_WAVES      4       // wavefronts per block of the per-lane kernel
+// launch shapes: work items of the per-lane kernel target LANE_ITEMS blocks
+// per launch with at least LANE_MIN_CHUNK kl candidates each; the team
+// kernel scans TEAM_KL_CHUNK candidates per work item.  Measured on the
+// 32-water def2-TZVP build (ms per build): team chunk 256/128/64/48/32 ->
+// 1280/1143/1109/1065/1077 with 2048 lane items; lane items 4096 -> 2048
+// at team chunk 256: 1280 -> 1236.
+#ifndef LANE_ITEMS
+#define LANE_ITEMS      2048
+#endif
+#ifndef LANE_MIN_CHUNK
+#define LANE_MIN_CHUNK  256
+#endif
+#ifndef TEAM_KL_CHUNK
+#define TEAM_KL_CHUNK   48
+#endif
 
 static thread_local char g_err[512];
 
@@ -1204,8 +1219,8 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
 {
     if constexpr (ClassTraits<LI, LJ, LK, LL>::LANE) {
         // whole kl lists per work item unless that leaves the GPU idle
-        int nchunks = (4096 + a.npairs_ij - 1) / a.npairs_ij;
-        int maxchunks = (a.npairs_kl + 255) / 256;
+        int nchunks = (LANE_ITEMS + a.npairs_ij - 1) / a.npairs_ij;
+        int maxchunks = (a.npairs_kl + LANE_MIN_CHUNK - 1) / LANE_MIN_CHUNK;
         if (nchunks > maxchunks) nchunks = maxchunks;
         if (nchunks < 1) nchunks = 1;
         const int kl_chunk = (a.npairs_kl + nchunks - 1) / nchunks;
@@ -1219,7 +1234,7 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
                            lds, stream, a, kl_chunk, use_lds);
     } else {
         using S = TeamShape<LI, LJ, LK, LL>;
-        const int kl_chunk = 256;
+        const int kl_chunk = TEAM_KL_CHUNK;
         int nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
