@@ -33,15 +33,36 @@
 // kernel scans TEAM_KL_CHUNK candidates per work item.  Measured on the
 // 32-water def2-TZVP build (ms per build): team chunk 256/128/64/48/32 ->
 // 1280/1143/1109/1065/1077 with 2048 lane items; lane items 4096 -> 2048
-// at team chunk 256: 1280 -> 1236.
+// at team chunk 256: 1280 -> 1236; without team K rows, lane items
+// 2048 -> 1024: 883 -> 873.
 #ifndef LANE_ITEMS
-#define LANE_ITEMS      2048
+#define LANE_ITEMS      1024
 #endif
 #ifndef LANE_MIN_CHUNK
 #define LANE_MIN_CHUNK  256
 #endif
 #ifndef TEAM_KL_CHUNK
 #define TEAM_KL_CHUNK   48
+#endif
+// LDS K rows of the team kernel (0 none, 1 K_il/K_jl) and its minimum
+// waves per SIMD (register budget).  With short work items zeroing and
+// flushing the rows costs more than the atomics they save: 1065 -> 884 ms
+// per build without them; a minimum of 2 or 3 waves per SIMD (fewer
+// registers) measured 1105 and 1197 ms.
+#ifndef TEAM_KROWS
+#define TEAM_KROWS      0
+#endif
+#ifndef TEAM_MIN_WAVES
+#define TEAM_MIN_WAVES  1
+#endif
+// LDS rows of the per-lane kernel: 2 = K_ik/K_jk and K_il/K_jl, 1 = K_il/K_jl
+// only; LANE_DROWS also stages the density rows (883 ms per build; 1061
+// with K_il/K_jl rows only, 887 without the density rows)
+#ifndef LANE_KROWS
+#define LANE_KROWS      2
+#endif
+#ifndef LANE_DROWS
+#define LANE_DROWS      1
 #endif
 
 static thread_local char g_err[512];
@@ -833,7 +854,7 @@ struct TeamShape {
 };
 
 template <int LI, int LJ, int LK, int LL>
-__global__ __launch_bounds__((TeamShape<LI, LJ, LK, LL>::TB))
+__global__ __launch_bounds__((TeamShape<LI, LJ, LK, LL>::TB), TEAM_MIN_WAVES)
 void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
 {
     using S = TeamShape<LI, LJ, LK, LL>;
@@ -1202,7 +1223,7 @@ static int krows_mode(const JKBuildArgs &a, size_t lds_static, int max_mode, boo
         return 2 | 4;
     }
     if (max_mode >= 2 && both + lds_static <= lds_limit()) { bytes = both; return 2; }
-    if (lonly + lds_static <= lds_limit()) { bytes = lonly; return 1; }
+    if (max_mode >= 1 && lonly + lds_static <= lds_limit()) { bytes = lonly; return 1; }
     return 0;
 }
 
@@ -1229,7 +1250,7 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
         size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * 2 * WAVE;
         size_t lds = 0;
-        const int use_lds = krows_mode<LI, LJ>(a, lds_static, 2, true, lds);
+        const int use_lds = krows_mode<LI, LJ>(a, lds_static, LANE_KROWS, LANE_DROWS, lds);
         hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(LANE_WAVES * WAVE),
                            lds, stream, a, kl_chunk, use_lds);
     } else {
@@ -1243,7 +1264,7 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
                             sizeof(double) * (S::NQ * (S::PER_Q + S::DSZ) + S::NFI * S::NFJ) +
                             256;
         size_t lds = 0;
-        const int use_lds = krows_mode<LI, LJ>(a, lds_static, 1, false, lds);
+        const int use_lds = krows_mode<LI, LJ>(a, lds_static, TEAM_KROWS, false, lds);
         hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB),
                            lds, stream, a, kl_chunk, use_lds);
     }
