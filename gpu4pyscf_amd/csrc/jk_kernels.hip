@@ -133,7 +133,33 @@ __device__ __forceinline__ double seg_sum(double v, int seg, int lane)
     return v;
 }
 
-// Rys nodes u_i = t_i^2 and weights on the Chebyshev tables.
+// Boys moments F_0..F_{NM-1}(x), x < xmax[1], by the 8-term Taylor series
+// about the nearest point x_g of the Boys grid:
+// F_m(x) = sum_k F_{m+k}(x_g) (x_g - x)^k / k!  (|x_g - x| <= step/2).
+template <int NM>
+__device__ __forceinline__ void boys_moments(const JKRysTable &tab, double x, double *f)
+{
+    constexpr double INV_FACT[8] = {1., 1., 1. / 2, 1. / 6, 1. / 24, 1. / 120, 1. / 720,
+                                    1. / 5040};
+    int g = (int)(x * tab.boys_inv_step + .5);
+    g = g < tab.boys_rows - 1 ? g : tab.boys_rows - 1;
+    const double d = g * tab.boys_step - x;
+    const double *F = tab.coef + tab.boys + (int64_t)g * JK_BOYS_M;
+    double Fg[NM + 7];
+#pragma unroll
+    for (int m = 0; m < NM + 7; ++m) Fg[m] = F[m];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        double s = Fg[m + 7] * INV_FACT[7];
+#pragma unroll
+        for (int k = 6; k >= 0; --k) s = fma(s, d, Fg[m + k] * INV_FACT[k]);
+        f[m] = s;
+    }
+}
+
+// Rys nodes u_i = t_i^2 and weights: the large-x limit above xmax[N-1];
+// below it, for one and two roots the Gauss rule of the Boys moments in
+// closed form, else the Chebyshev tables.
 template <int N>
 __device__ __forceinline__ void rys_roots(const JKRysTable &tab, double x,
                                           double *r, double *w)
@@ -145,6 +171,29 @@ __device__ __forceinline__ void rys_roots(const JKRysTable &tab, double x,
         double sx = sqrt(ix);
 #pragma unroll
         for (int i = 0; i < N; ++i) { r[i] = lr[i] * ix; w[i] = lw[i] * sx; }
+        return;
+    }
+    if constexpr (N == 1) {
+        double f[2];
+        boys_moments<2>(tab, x, f);
+        w[0] = f[0];
+        r[0] = f[1] / f[0];
+        return;
+    } else if constexpr (N == 2) {
+        // monic p(u) = u^2 + c1 u + c0 orthogonal to 1 and u under the
+        // moments: F2 + c1 F1 + c0 F0 = 0, F3 + c1 F2 + c0 F1 = 0
+        double f[4];
+        boys_moments<4>(tab, x, f);
+        const double idet = 1. / (f[1] * f[1] - f[0] * f[2]);
+        const double c1 = (f[0] * f[3] - f[1] * f[2]) * idet;
+        const double c0 = (f[2] * f[2] - f[1] * f[3]) * idet;
+        const double disc = sqrt(c1 * c1 - 4. * c0);
+        const double q = -.5 * (c1 + copysign(disc, c1));
+        const double ua = q, ub = c0 / q;
+        const double u1 = fmin(ua, ub), u2 = fmax(ua, ub);
+        const double w1 = (f[1] - u2 * f[0]) / (u1 - u2);
+        r[0] = u1; r[1] = u2;
+        w[0] = w1; w[1] = f[0] - w1;
         return;
     }
     const int D1 = tab.degree + 1;
@@ -174,6 +223,15 @@ __device__ __forceinline__ void rys_root_one(const JKRysTable &tab, double x, in
         double ix = 1. / x;
         r = tab.coef[tab.large_r + N * (N - 1) / 2 + i] * ix;
         w = tab.coef[tab.large_w + N * (N - 1) / 2 + i] * sqrt(ix);
+        return;
+    }
+    if constexpr (N <= 2) {
+        double rr[N], ww[N];
+        rys_roots<N>(tab, x, rr, ww);
+        r = rr[0]; w = ww[0];
+#pragma unroll
+        for (int k = 1; k < N; ++k)
+            if (i == k) { r = rr[k]; w = ww[k]; }
         return;
     }
     const int D1 = tab.degree + 1;

@@ -14,8 +14,9 @@ _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, 'libjkhip.so')
 TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
 
-JK_ABI_VERSION = 4
+JK_ABI_VERSION = 5
 JK_RYS_NMAX = 11
+JK_BOYS_M = 11
 JK_LMAX = 3
 
 c_void_p = ctypes.c_void_p
@@ -30,6 +31,10 @@ class JKRysTable(ctypes.Structure):
         ('large_r', ctypes.c_int64),
         ('large_w', ctypes.c_int64),
         ('degree', ctypes.c_int),
+        ('boys_rows', ctypes.c_int),
+        ('boys', ctypes.c_int64),
+        ('boys_step', ctypes.c_double),
+        ('boys_inv_step', ctypes.c_double),
     ]
 
 
@@ -117,9 +122,13 @@ def load_rys_table_host():
     coef = t['coef']
     lr = t['large_r']
     lw = t['large_w']
-    data = np.concatenate([coef, lr, lw]).astype(np.float64)
+    boys = np.ascontiguousarray(t['boys'], dtype=np.float64)
+    assert boys.shape[1] == JK_BOYS_M
+    data = np.concatenate([coef, lr, lw, boys.ravel()]).astype(np.float64)
     meta = dict(off=t['off'].astype(np.int64), xmax=t['xmax'].astype(np.float64),
-                large_r=len(coef), large_w=len(coef) + len(lr), degree=int(t['degree']))
+                large_r=len(coef), large_w=len(coef) + len(lr), degree=int(t['degree']),
+                boys=len(coef) + len(lr) + len(lw), boys_rows=boys.shape[0],
+                boys_step=float(t['boys_step']))
     return data, meta
 
 
@@ -144,5 +153,9 @@ def rys_table(device):
         tab.large_r = meta['large_r']
         tab.large_w = meta['large_w']
         tab.degree = meta['degree']
+        tab.boys = meta['boys']
+        tab.boys_rows = meta['boys_rows']
+        tab.boys_step = meta['boys_step']
+        tab.boys_inv_step = 1. / meta['boys_step']
         _tables[key] = ((buf, buf32), tab)
     return _tables[key]

@@ -24,14 +24,17 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  4
+#define JK_ABI_VERSION  5
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         3       /* max angular momentum on the GPU path */
 
 /* Chebyshev tables of Rys roots/weights (device pointer + layout).
  * coef[off[n-1] + (k*2n + f)*(degree+1) + d] on unit intervals [k, k+1),
  * k < xmax[n-1]; above, u_i = large[n*(n-1)/2 + i] / x and
- * w_i = large[large_w + n*(n-1)/2 + i] / sqrt(x). */
+ * w_i = large[large_w + n*(n-1)/2 + i] / sqrt(x).
+ * Boys grid (one and two roots): coef[boys + g*JK_BOYS_M + m] = F_m(g*boys_step),
+ * boys_rows grid points covering x < xmax[1]. */
+#define JK_BOYS_M       11
 typedef struct {
     const double *coef;
     const float *coef32;        /* coef rounded to float (FP32 quartets) */
@@ -40,6 +43,9 @@ typedef struct {
     int64_t large_r;            /* offset of large-x nodes in coef */
     int64_t large_w;            /* offset of large-x weights in coef */
     int degree;
+    int boys_rows;
+    int64_t boys;               /* offset of the Boys grid in coef */
+    double boys_step, boys_inv_step;
 } JKRysTable;
 
 /* One angular-class task of the J/K build.

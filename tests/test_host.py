@@ -33,7 +33,7 @@ def test_library_exports_header_symbols():
 def test_struct_layout_matches_header():
     from gpu4pyscf_amd.lib import clib
     # JKRysTable: 2 pointers + 11 int64 + 11 double + 2 int64 + int (+pad)
-    assert ctypes.sizeof(clib.JKRysTable) == 16 + 88 + 88 + 16 + 8
+    assert ctypes.sizeof(clib.JKRysTable) == 16 + 88 + 88 + 16 + 8 + 8 + 16
     assert clib.JKBuildArgs.rys.offset % 8 == 0
 
 

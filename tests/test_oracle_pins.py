@@ -63,6 +63,60 @@ def test_product_table_matches_oracle():
             assert abs(w - w0).max() <= 5e-14 * abs(w0).max()
 
 
+def test_boys_grid_matches_incomplete_gamma():
+    '''Boys grid of the product table: F_m(x_g) from mpmath's incomplete
+    gamma function, an evaluation independent of the generator's series.'''
+    t = np.load('gpu4pyscf_amd/csrc/rys_table.npz')
+    boys, step = t['boys'], float(t['boys_step'])
+    assert boys.shape[1] == 11 and (boys.shape[0] - 1) * step >= t['xmax'][1]
+    for g in (0, 1, 7, 100, 233, 389, boys.shape[0] - 1):
+        for m in (0, 1, 3, 6, 10):
+            ref = float(_boys(m, g * step))
+            assert abs(boys[g, m] - ref) <= 1e-15 * ref
+
+
+def test_boys_grid_roots_match_oracle():
+    '''One- and two-root Gauss rules as the per-lane kernels form them: the
+    8-term Taylor series of the Boys moments about the nearest grid point,
+    then the closed-form rule of the moments, against the oracle's roots.'''
+    t = np.load('gpu4pyscf_amd/csrc/rys_table.npz')
+    boys, step, xmax = t['boys'], float(t['boys_step']), t['xmax']
+    inv_fact = [1. / math.factorial(k) for k in range(8)]
+
+    def moments(x, nm):
+        g = min(int(x / step + .5), boys.shape[0] - 1)
+        d = g * step - x
+        out = []
+        for m in range(nm):
+            s = boys[g, m + 7] * inv_fact[7]
+            for k in range(6, -1, -1):
+                s = s * d + boys[g, m + k] * inv_fact[k]
+            out.append(s)
+        return out
+
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([[0., 1e-9, .05, .0500001, 38.95, 46.95],
+                         rng.random(40) * 47., rng.random(20)])
+    for x in xs:
+        if x < xmax[0]:
+            f0, f1 = moments(x, 2)
+            u0, w0 = O.rys_roots(1, x)
+            assert abs(f1 / f0 - u0[0]) <= 1e-14 * u0[0]
+            assert abs(f0 - w0[0]) <= 1e-14 * w0[0]
+        if x < xmax[1]:
+            f0, f1, f2, f3 = moments(x, 4)
+            det = f1 * f1 - f0 * f2
+            c1 = (f0 * f3 - f1 * f2) / det
+            c0 = (f2 * f2 - f1 * f3) / det
+            disc = math.sqrt(c1 * c1 - 4 * c0)
+            q = -.5 * (c1 + math.copysign(disc, c1))
+            u1, u2 = sorted((q, c0 / q))
+            w1 = (f1 - u2 * f0) / (u1 - u2)
+            u0, w0 = O.rys_roots(2, x)
+            assert abs(np.array([u1, u2]) - u0).max() <= 2e-14 * abs(u0).max()
+            assert abs(np.array([w1, f0 - w1]) - w0).max() <= 2e-14 * abs(w0).max()
+
+
 def test_ssss_closed_form():
     '''(ss|ss) of unnormalised-free s Gaussians: 2 pi^2.5/(pq sqrt(p+q)) K F0.'''
     from gpu4pyscf_amd.gto.mole import M
