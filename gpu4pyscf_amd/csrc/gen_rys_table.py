@@ -27,7 +27,7 @@ unit intervals [k, k+1).  Output: rys_table.npz with
   coef[off[n-1] + (k*2n + f)*(DEGREE+1) + d]   f = 2i (root i), 2i+1 (weight i)
   xmax[n-1] = X_n, off[n-1], large_r[n*(n-1)/2 + i], large_w[...].
 
-For one and two roots the kernels instead take the moments F_0..F_{2n-1}
+For one to three roots the kernels instead take the moments F_0..F_{2n-1}
 from a Boys grid, boys[g, m] = F_m(g * BOYS_STEP) for m <= BOYS_M - 1, by
 the BOYS_TERMS-term Taylor series F_m(x) = sum_k F_{m+k}(x_g) (x_g - x)^k / k!
 about the nearest grid point, and form the Gauss rule from the moments.
@@ -45,7 +45,7 @@ DEGREE = 11
 WIDTH = 1.0
 BOYS_STEP = 0.1
 BOYS_TERMS = 8
-BOYS_M = 3 + BOYS_TERMS        # F_0 .. F_10: moments of up to two roots
+BOYS_M = 5 + BOYS_TERMS        # F_0 .. F_12: moments of up to three roots
 mp.mp.dps = 40
 
 
@@ -187,8 +187,8 @@ def main(out):
             large_r.extend([float(v) for v in lr])
             large_w.extend([float(v) for v in lw])
             print(f'n={n} xmax={X} intervals={nint}', flush=True)
-        # Boys grid covering x < X_2 (nearest grid point of any such x)
-        rows = int(xmax[1] / BOYS_STEP) + 2
+        # Boys grid covering x < X_3 (nearest grid point of any such x)
+        rows = int(xmax[2] / BOYS_STEP) + 2
         boys_grid = np.array([[float(v) for v in boys(BOYS_M - 1, g * BOYS_STEP)]
                               for g in range(rows)])
     np.savez(out, coef=np.concatenate(coefs), off=np.asarray(offs, dtype=np.int64),

@@ -133,7 +133,7 @@ __device__ __forceinline__ double seg_sum(double v, int seg, int lane)
     return v;
 }
 
-// Boys moments F_0..F_{NM-1}(x), x < xmax[1], by the 8-term Taylor series
+// Boys moments F_0..F_{NM-1}(x), x < xmax[2], by the 8-term Taylor series
 // about the nearest point x_g of the Boys grid:
 // F_m(x) = sum_k F_{m+k}(x_g) (x_g - x)^k / k!  (|x_g - x| <= step/2).
 template <int NM>
@@ -158,8 +158,8 @@ __device__ __forceinline__ void boys_moments(const JKRysTable &tab, double x, do
 }
 
 // Rys nodes u_i = t_i^2 and weights: the large-x limit above xmax[N-1];
-// below it, for one and two roots the Gauss rule of the Boys moments in
-// closed form, else the Chebyshev tables.
+// below it, for one to three roots the Gauss rule of the Boys moments,
+// else the Chebyshev tables.
 template <int N>
 __device__ __forceinline__ void rys_roots(const JKRysTable &tab, double x,
                                           double *r, double *w)
@@ -194,6 +194,51 @@ __device__ __forceinline__ void rys_roots(const JKRysTable &tab, double x,
         const double w1 = (f[1] - u2 * f[0]) / (u1 - u2);
         r[0] = u1; r[1] = u2;
         w[0] = w1; w[1] = f[0] - w1;
+        return;
+    } else if constexpr (N == 3) {
+        // recurrence coefficients of the monic orthogonal polynomials from
+        // the moments (Chebyshev's algorithm, Gautschi 2004, Alg. 2.1), the
+        // nodes from Viete's trigonometric solution of the characteristic
+        // cubic of the Jacobi matrix refined by two Newton steps on the
+        // three-term recurrence, the weights by the Christoffel formula
+        double mu[6];
+        boys_moments<6>(tab, x, mu);
+        const double a0 = mu[1] / mu[0], b0 = mu[0];
+        const double n1 = mu[2] - a0 * mu[1], n2 = mu[3] - a0 * mu[2];
+        const double n3 = mu[4] - a0 * mu[3], n4 = mu[5] - a0 * mu[4];
+        const double a1 = n2 / n1 - a0, b1 = n1 / mu[0];
+        const double m2 = n3 - a1 * n2 - b1 * mu[2], m3 = n4 - a1 * n3 - b1 * mu[3];
+        const double a2 = m3 / m2 - n2 / n1, b2 = m2 / n1;
+        const double c2 = -(a0 + a1 + a2);
+        const double c1 = a0 * a1 + a0 * a2 + a1 * a2 - b1 - b2;
+        const double c0 = b2 * a0 + b1 * a2 - a0 * a1 * a2;
+        const double p = c1 - c2 * c2 * (1. / 3.);
+        const double q = c2 * c2 * c2 * (2. / 27.) - c2 * c1 * (1. / 3.) + c0;
+        const double rr = sqrt(-p * (1. / 3.));
+        const double phi = acos(fmin(1., fmax(-1., -q / (2. * rr * rr * rr))));
+        double u[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            u[k] = 2. * rr * cos((phi - 2. * M_PI * k) * (1. / 3.)) - c2 * (1. / 3.);
+        const double nrm1 = b0 * b1, nrm2 = nrm1 * b2;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double v = u[k];
+#pragma unroll
+            for (int it = 0; it < 2; ++it) {
+                const double p1 = v - a0;
+                const double p2 = (v - a1) * p1 - b1, d2 = p1 + (v - a1);
+                const double p3 = (v - a2) * p2 - b2 * p1, d3 = p2 + (v - a2) * d2 - b2;
+                v -= p3 / d3;
+            }
+            const double p1 = v - a0, p2 = (v - a1) * p1 - b1;
+            w[k] = 1. / (1. / b0 + p1 * p1 / nrm1 + p2 * p2 / nrm2);
+            u[k] = v;
+        }
+        // ascending order (cos is decreasing on [0, pi]: k = 0 is the largest)
+        r[0] = u[2]; r[1] = u[1]; r[2] = u[0];
+        const double w0 = w[0];
+        w[0] = w[2]; w[2] = w0;
         return;
     }
     const int D1 = tab.degree + 1;

@@ -14,9 +14,9 @@ _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, 'libjkhip.so')
 TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
 
-JK_ABI_VERSION = 5
+JK_ABI_VERSION = 6
 JK_RYS_NMAX = 11
-JK_BOYS_M = 11
+JK_BOYS_M = 13
 JK_LMAX = 3
 
 c_void_p = ctypes.c_void_p

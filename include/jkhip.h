@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  5
+#define JK_ABI_VERSION  6
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         3       /* max angular momentum on the GPU path */
 
@@ -32,9 +32,9 @@ extern "C" {
  * coef[off[n-1] + (k*2n + f)*(degree+1) + d] on unit intervals [k, k+1),
  * k < xmax[n-1]; above, u_i = large[n*(n-1)/2 + i] / x and
  * w_i = large[large_w + n*(n-1)/2 + i] / sqrt(x).
- * Boys grid (one and two roots): coef[boys + g*JK_BOYS_M + m] = F_m(g*boys_step),
- * boys_rows grid points covering x < xmax[1]. */
-#define JK_BOYS_M       11
+ * Boys grid (one to three roots): coef[boys + g*JK_BOYS_M + m] = F_m(g*boys_step),
+ * boys_rows grid points covering x < xmax[2]. */
+#define JK_BOYS_M       13
 typedef struct {
     const double *coef;
     const float *coef32;        /* coef rounded to float (FP32 quartets) */

@@ -68,17 +68,19 @@ def test_boys_grid_matches_incomplete_gamma():
     gamma function, an evaluation independent of the generator's series.'''
     t = np.load('gpu4pyscf_amd/csrc/rys_table.npz')
     boys, step = t['boys'], float(t['boys_step'])
-    assert boys.shape[1] == 11 and (boys.shape[0] - 1) * step >= t['xmax'][1]
+    assert boys.shape[1] == 13 and (boys.shape[0] - 1) * step >= t['xmax'][2]
     for g in (0, 1, 7, 100, 233, 389, boys.shape[0] - 1):
-        for m in (0, 1, 3, 6, 10):
+        for m in (0, 1, 3, 6, 10, 12):
             ref = float(_boys(m, g * step))
             assert abs(boys[g, m] - ref) <= 1e-15 * ref
 
 
 def test_boys_grid_roots_match_oracle():
-    '''One- and two-root Gauss rules as the per-lane kernels form them: the
+    '''One- to three-root Gauss rules as the per-lane kernels form them: the
     8-term Taylor series of the Boys moments about the nearest grid point,
-    then the closed-form rule of the moments, against the oracle's roots.'''
+    then the rule of the moments (closed form for one and two roots;
+    Chebyshev's algorithm, Viete's cubic and two Newton steps for three),
+    against the oracle's roots.'''
     t = np.load('gpu4pyscf_amd/csrc/rys_table.npz')
     boys, step, xmax = t['boys'], float(t['boys_step']), t['xmax']
     inv_fact = [1. / math.factorial(k) for k in range(8)]
@@ -95,8 +97,8 @@ def test_boys_grid_roots_match_oracle():
         return out
 
     rng = np.random.default_rng(3)
-    xs = np.concatenate([[0., 1e-9, .05, .0500001, 38.95, 46.95],
-                         rng.random(40) * 47., rng.random(20)])
+    xs = np.concatenate([[0., 1e-9, .05, .0500001, 38.95, 46.95, 53.95],
+                         rng.random(40) * 54., rng.random(20)])
     for x in xs:
         if x < xmax[0]:
             f0, f1 = moments(x, 2)
@@ -115,6 +117,36 @@ def test_boys_grid_roots_match_oracle():
             u0, w0 = O.rys_roots(2, x)
             assert abs(np.array([u1, u2]) - u0).max() <= 2e-14 * abs(u0).max()
             assert abs(np.array([w1, f0 - w1]) - w0).max() <= 2e-14 * abs(w0).max()
+        if x < xmax[2]:
+            mu = moments(x, 6)
+            a0, b0 = mu[1] / mu[0], mu[0]
+            n1, n2 = mu[2] - a0 * mu[1], mu[3] - a0 * mu[2]
+            n3, n4 = mu[4] - a0 * mu[3], mu[5] - a0 * mu[4]
+            a1, b1 = n2 / n1 - a0, n1 / mu[0]
+            m2, m3 = n3 - a1 * n2 - b1 * mu[2], n4 - a1 * n3 - b1 * mu[3]
+            a2, b2 = m3 / m2 - n2 / n1, m2 / n1
+            c2 = -(a0 + a1 + a2)
+            c1 = a0 * a1 + a0 * a2 + a1 * a2 - b1 - b2
+            c0 = b2 * a0 + b1 * a2 - a0 * a1 * a2
+            p = c1 - c2 * c2 / 3
+            q = 2 * c2**3 / 27 - c2 * c1 / 3 + c0
+            rr = math.sqrt(-p / 3)
+            phi = math.acos(min(1., max(-1., -q / (2 * rr**3))))
+            us, ws = [], []
+            for k in range(3):
+                v = 2 * rr * math.cos((phi - 2 * math.pi * k) / 3) - c2 / 3
+                for _ in range(2):
+                    p1 = v - a0
+                    p2, d2 = (v - a1) * p1 - b1, p1 + (v - a1)
+                    p3, d3 = (v - a2) * p2 - b2 * p1, p2 + (v - a2) * d2 - b2
+                    v -= p3 / d3
+                p1 = v - a0
+                p2 = (v - a1) * p1 - b1
+                ws.append(1 / (1 / b0 + p1 * p1 / (b0 * b1) + p2 * p2 / (b0 * b1 * b2)))
+                us.append(v)
+            u0, w0 = O.rys_roots(3, x)
+            assert abs(np.array(us[::-1]) - u0).max() <= 3e-14 * abs(u0).max()
+            assert abs(np.array(ws[::-1]) - w0).max() <= 3e-14 * abs(w0).max()
 
 
 def test_ssss_closed_form():
