@@ -27,6 +27,18 @@
 #define Q_COND_MARGIN   4.f
 #define WAVE            64
 #define RYS_DEG_MAX     15
+// the team kernel's one-root-per-lane evaluation uses the Boys-grid rule
+// (all roots, one kept) up to this many roots, the Chebyshev tables above
+#ifndef ROOT_ONE_BOYS_NMAX
+#define ROOT_ONE_BOYS_NMAX 2
+#endif
+// classes whose Cartesian block plus three 2D tables fit in LANE_MAX
+// doubles run one quartet per lane, the others on teams of lanes.
+// Measured (ms per 32-water def2-TZVP build): 120 -> 720, 150 -> 679,
+// 170 -> 660 (adds the (dd|ps)-sized classes), 240 -> 682.
+#ifndef LANE_MAX
+#define LANE_MAX        170
+#endif
 #define LANE_WAVES      4       // wavefronts per block of the per-lane kernel
 // launch shapes: work items of the per-lane kernel target LANE_ITEMS blocks
 // per launch with at least LANE_MIN_CHUNK kl candidates each; the team
@@ -270,7 +282,7 @@ __device__ __forceinline__ void rys_root_one(const JKRysTable &tab, double x, in
         w = tab.coef[tab.large_w + N * (N - 1) / 2 + i] * sqrt(ix);
         return;
     }
-    if constexpr (N <= 2) {
+    if constexpr (N <= ROOT_ONE_BOYS_NMAX) {
         double rr[N], ww[N];
         rys_roots<N>(tab, x, rr, ww);
         r = rr[0]; w = ww[0];
@@ -606,7 +618,16 @@ __device__ __forceinline__ void eri_lane(T *gout, const double *pij, int nij,
             T theta = aij * akl * r_aa;
             T fac = (T)p[5] * Kkl * sqrt(r_aa);
             T rt[N], wt[N];
-            rys_roots<N>(rys, theta * rr, rt, wt);
+            if constexpr (sizeof(T) == sizeof(float) && N <= 3) {
+                // the Boys-grid rule in FP64 is cheaper than the float
+                // Chebyshev series for up to three roots
+                double rd[N], wd[N];
+                rys_roots<N>(rys, (double)(theta * rr), rd, wd);
+#pragma unroll
+                for (int i = 0; i < N; ++i) { rt[i] = (T)rd[i]; wt[i] = (T)wd[i]; }
+            } else {
+                rys_roots<N>(rys, theta * rr, rt, wt);
+            }
 #pragma unroll
             for (int ir = 0; ir < N; ++ir) {
                 T rt_aa = rt[ir] * r_aa;
@@ -1334,8 +1355,8 @@ template <int LI, int LJ, int LK, int LL>
 struct ClassTraits {
     static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
     static constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
-    // per-lane path when the block and three 2D tables fit in ~120 doubles
-    static constexpr bool LANE = NF + 3 * GS <= 120;
+    // per-lane path when the block and three 2D tables fit in LANE_MAX doubles
+    static constexpr bool LANE = NF + 3 * GS <= LANE_MAX;
 };
 
 template <int LI, int LJ, int LK, int LL>

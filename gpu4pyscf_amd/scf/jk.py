@@ -75,14 +75,17 @@ def group_by_split_points(q, split_points):
     return np.concatenate(subsets).astype(np.int64)
 
 
+LANE_MAX = 170      # LANE_MAX of csrc/jk_kernels.hip
+
+
 def class_kernel_name(ls):
     '''Name of the HIP kernel template that evaluates angular class ls =
     (li, lj, lk, ll): the per-lane kernel when the Cartesian block and the
-    three 2D tables fit in 120 doubles, else the team kernel.'''
+    three 2D tables fit in LANE_MAX doubles, else the team kernel.'''
     nc = [(l + 1) * (l + 2) // 2 for l in ls]
     nf = nc[0] * nc[1] * nc[2] * nc[3]
     gs = (ls[0] + 1) * (ls[1] + 1) * (ls[2] + 1) * (ls[3] + 1)
-    kind = 'jk_lane_kernel' if nf + 3 * gs <= 120 else 'jk_team_kernel'
+    kind = 'jk_lane_kernel' if nf + 3 * gs <= LANE_MAX else 'jk_team_kernel'
     return f'{kind}<{ls[0]}, {ls[1]}, {ls[2]}, {ls[3]}>'
 
 
