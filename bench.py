@@ -84,8 +84,10 @@ def scf_leg(mol, fp32_tol, guess_model=None, conv_tol=1e-8):
     import torch
     from gpu4pyscf_amd.scf import jk
     from gpu4pyscf_amd.scf.hf import RHF
+    from gpu4pyscf_amd.scf.atom_guess import atom_guess
     res = {}
     opt = jk._VHFOpt(mol).build()      # screening setup shared by every variant
+    atom_guess(mol)                    # per-element atomic densities, cached
     variants = [('fp64', None, 'atom'), ('mixed', fp32_tol, 'atom')]
     if guess_model:
         from gpu4pyscf_amd.guess.learned import LearnedGuess

@@ -58,33 +58,45 @@ def int1e(mol, vhfopt):
 
 
 class CDIIS:
-    '''Commutator DIIS on the error FDS - SDF in the orthonormal basis.'''
+    '''Commutator DIIS on the error FDS - SDF in the orthonormal basis.
+    The overlap matrix of the stored error vectors is kept incrementally:
+    one update adds the dot products of the new vector with the stored ones
+    (elementwise products and reductions on the device).'''
 
     def __init__(self, space=8):
         self.space = space
         self.fs = []
         self.es = []
+        self.B = np.zeros((0, 0))
 
     def update(self, f, e):
         torch = _torch()
-        self.fs.append(f)
-        self.es.append(e)
-        if len(self.fs) > self.space:
+        e = e.reshape(-1)
+        if len(self.fs) == self.space:
             self.fs.pop(0)
             self.es.pop(0)
-        n = len(self.fs)
-        B = torch.empty((n + 1, n + 1), dtype=torch.float64, device=f.device)
-        E = torch.stack([x.reshape(-1) for x in self.es])
-        B[:n, :n] = E @ E.T
-        B[n, :n] = B[:n, n] = -1.
-        B[n, n] = 0.
-        rhs = torch.zeros(n + 1, dtype=torch.float64, device=f.device)
+            self.B = self.B[1:, 1:]
+        dots = torch.stack([torch.sum(x * e) for x in self.es] + [torch.sum(e * e)])
+        dots = dots.cpu().numpy()
+        n = len(self.es) + 1
+        B = np.empty((n, n))
+        B[:n - 1, :n - 1] = self.B
+        B[n - 1, :] = B[:, n - 1] = dots
+        self.B = B
+        self.fs.append(f)
+        self.es.append(e)
+        # the (n+1) x (n+1) system is solved on the host
+        A = np.empty((n + 1, n + 1))
+        A[:n, :n] = B
+        A[n, :n] = A[:n, n] = -1.
+        A[n, n] = 0.
+        rhs = np.zeros(n + 1)
         rhs[n] = -1.
         try:
-            c = torch.linalg.solve(B, rhs)
-        except RuntimeError:
-            c = torch.linalg.lstsq(B.cpu(), rhs.cpu()).solution.to(f.device)
-        return sum(c[i] * self.fs[i] for i in range(n))
+            c = np.linalg.solve(A, rhs)
+        except np.linalg.LinAlgError:
+            c = np.linalg.lstsq(A, rhs, rcond=None)[0]
+        return sum(float(c[i]) * self.fs[i] for i in range(n))
 
 
 class RHF:
@@ -106,6 +118,9 @@ class RHF:
     # a callable (learned) guess is projected on the closest idempotent
     # density with the right electron count before the first Fock build
     project_guess = True
+    # accumulate per-phase wall times (jk, eig, diis) in self.timings; the
+    # device synchronisations this needs cost a little time
+    time_phases = False
 
     def __init__(self, mol):
         self.mol = mol
@@ -219,9 +234,23 @@ class RHF:
         raise ValueError(f'unknown init_guess {key!r}')
 
     # -- driver -------------------------------------------------------------
+    def _timed(self, key, fn, *args):
+        '''fn(*args), adding its wall time (device synchronised) to
+        self.timings[key] when self.time_phases is set.'''
+        if not self.time_phases:
+            return fn(*args)
+        torch = _torch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn(*args)
+        torch.cuda.synchronize()
+        self.timings[key] = self.timings.get(key, 0.) + time.perf_counter() - t0
+        return out
+
     def kernel(self, dm0=None):
         torch = _torch()
         t_start = time.perf_counter()
+        self.timings = {}
         mol = self.mol
         e_nuc = mol.energy_nuc()
         h1e, S = self.get_hcore_ovlp()
@@ -230,7 +259,7 @@ class RHF:
         dm = self.get_init_guess(h1e, S, X, dm0 if dm0 is not None else None)
         t_guess = time.perf_counter()
         self._fp32_now = None           # the full-density build is FP64
-        vhf = self.get_veff(dm)
+        vhf = self._timed('jk', self.get_veff, dm)
         self._fp32_now = self.fp32_tol  # incremental builds may use FP32
         e_tot = float(torch.sum(dm * (h1e + .5 * vhf))) + e_nuc
         diis = CDIIS(self.diis_space)
@@ -245,11 +274,11 @@ class RHF:
             if cycle >= self.diis_start_cycle:
                 sdf = S @ dm @ fock
                 err = X.T @ (sdf.T - sdf) @ X
-                fock = diis.update(fock, err)
-            mo_energy, mo_coeff = self.eig(fock, X)
+                fock = self._timed('diis', diis.update, fock, err)
+            mo_energy, mo_coeff = self._timed('eig', self.eig, fock, X)
             mo_occ = self.get_occ(mo_energy)
             dm = self.make_rdm1(mo_coeff, mo_occ)
-            vhf = self.get_veff(dm, dm_last, vhf)
+            vhf = self._timed('jk', self.get_veff, dm, dm_last, vhf)
             e_tot = float(torch.sum(dm * (h1e + .5 * vhf))) + e_nuc
             fock = h1e + vhf
             # orbital gradient: occupied-virtual block of the MO Fock matrix
@@ -275,7 +304,7 @@ class RHF:
         self.mo_energy, self.mo_coeff, self.mo_occ = mo_energy, mo_coeff, mo_occ
         self.dm = dm
         torch.cuda.synchronize()
-        self.timings = dict(guess=t_guess - t_start, total=time.perf_counter() - t_start)
+        self.timings.update(guess=t_guess - t_start, total=time.perf_counter() - t_start)
         return e_tot
 
     scf = kernel
