@@ -163,9 +163,17 @@ def main():
         shape, spacing, basis = (2, 2, 2), 3.0, 'def2-svp'
     elif args.config == 'water64-tzvp':
         shape, spacing, basis = (4, 4, 4), 3.1, 'def2-tzvp'
+    elif args.config == 'alkane30-tzvp':
+        shape, spacing, basis = None, None, 'def2-tzvp'
     else:
         raise SystemExit(f'unknown config {args.config}')
-    mol = water_mol(shape, spacing, basis)
+    if shape is None:
+        # all-trans C30H62, N(0, 0.02 A) jitter, seed 0
+        from gpu4pyscf_amd.gto import geometry as G
+        from gpu4pyscf_amd.gto.mole import M
+        mol = M(atom=G.to_string(G.alkane(30, seed=0)), basis=basis)
+    else:
+        mol = water_mol(shape, spacing, basis)
     log(f'[bench] {args.config}: nao={mol.nao} nbas={mol.nbas}')
     dev = torch.device('cuda', 0)
     t0 = time.perf_counter()
@@ -273,10 +281,12 @@ def main():
                roofline=roofline, setup_s=t_build, mixed=mixed)
     if not args.no_scf:
         model = args.guess_model
-        if model == 'auto':
+        if model == 'auto' and args.config.startswith('water'):
             data = os.path.join(ROOT, 'gpu4pyscf_amd', 'guess', 'data')
             model = os.path.join(data, {'def2-tzvp': 'water16_def2tzvp.npz',
                                         'def2-svp': 'water8_def2svp.npz'}[basis])
+        elif model == 'auto':
+            model = None            # the shipped weights were trained on water
         elif model == 'none':
             model = None
         out['scf'] = scf_leg(mol, args.fp32_tol, model)
