@@ -39,7 +39,9 @@
 #ifndef LANE_MAX
 #define LANE_MAX        170
 #endif
+#ifndef LANE_WAVES
 #define LANE_WAVES      4       // wavefronts per block of the per-lane kernel
+#endif
 // launch shapes: work items of the per-lane kernel target LANE_ITEMS blocks
 // per launch with at least LANE_MIN_CHUNK kl candidates each; the team
 // kernel scans TEAM_KL_CHUNK candidates per work item.  Measured on the
