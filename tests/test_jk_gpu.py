@@ -91,3 +91,20 @@ def test_mixed_precision_incremental_jk():
     err = max(float((vj1 - vj0).abs().max()), float((vk1 - vk0).abs().max()))
     print(f'fp32 fraction {frac:.3f}, max |dJ|,|dK| = {err:.2e}')
     assert err < 1e-9
+
+
+def test_alkane_tzvp_jk_matches_oracle():
+    '''C4H10 (all-trans, jittered) in def2-TZVP: the C and H shells of the
+    alkane workload, J/K of a random symmetric density against the oracle.'''
+    from gpu4pyscf_amd.gto import geometry as G
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.scf import jk
+    mol = M(atom=G.to_string(G.alkane(4, seed=1)), basis='def2-tzvp')
+    rng = np.random.default_rng(4)
+    c = np.linalg.qr(rng.standard_normal((mol.nao, mol.nelectron // 2)))[0]
+    dm = 2 * c @ c.T
+    print(f'[alkane C4H10] nao={mol.nao}', flush=True)
+    vj, vk = jk.get_jk(mol, dm, hermi=1)
+    rj, rk = O.get_jk(mol, dm, hermi=1)
+    assert abs(vj - rj).max() < 1e-10
+    assert abs(vk - rk).max() < 1e-10
