@@ -39,6 +39,12 @@
 #ifndef LANE_MAX
 #define LANE_MAX        170
 #endif
+// a block size the per-lane kernel leaves to the team kernel (0: none);
+// sending the spilling (ff|ss)/(fs|fs) (100) or (dd|ps)-sized (108) classes
+// back to the team kernel measured 658 and 670 ms against 652
+#ifndef LANE_SKIP_NF
+#define LANE_SKIP_NF    0
+#endif
 #ifndef LANE_WAVES
 #define LANE_WAVES      4       // wavefronts per block of the per-lane kernel
 #endif
@@ -1358,7 +1364,7 @@ struct ClassTraits {
     static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
     static constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
     // per-lane path when the block and three 2D tables fit in LANE_MAX doubles
-    static constexpr bool LANE = NF + 3 * GS <= LANE_MAX;
+    static constexpr bool LANE = NF + 3 * GS <= LANE_MAX && NF != LANE_SKIP_NF;
 };
 
 template <int LI, int LJ, int LK, int LL>
