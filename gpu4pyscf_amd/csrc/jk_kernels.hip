@@ -14,9 +14,16 @@
 //     2D integrals of all roots staged in LDS, each lane owning one (i,j) row
 //     of the block, the block staged through LDS for the contraction.
 // Digestion follows gpu4pyscf/lib/gvhf-rys/rys_contract_jk.cu:291-330 (four K
-// blocks and two J blocks per quartet, symmetry factors of :156-158,214-220),
-// with FP64 atomics into HBM; the J_ij block, common to a work item, is
-// reduced across the wavefront before its atomics.
+// blocks and two J blocks per quartet, symmetry factors of :156-158,214-220).
+// On this 8-XCD part an FP64 atomic add into HBM is resolved on the memory
+// side, so the kernels reduce in registers and LDS first: J_ij (common to a
+// work item) across the wavefront or in the row-owning lane, the K rows of
+// the work item's i and j AOs in LDS (per-lane kernel), the density blocks
+// read by the digestion staged in LDS; what remains per quartet (J_kl, and
+// K in the team kernel) goes to HBM with FP64 atomics.
+// Rys roots: for up to three roots the Gauss rule of Boys moments taken from
+// a Boys grid by a Taylor series; otherwise Chebyshev tables; both generated
+// by gen_rys_table.py.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
