@@ -244,7 +244,9 @@ class RHF:
         t0 = time.perf_counter()
         out = fn(*args)
         torch.cuda.synchronize()
-        self.timings[key] = self.timings.get(key, 0.) + time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        self.timings[key] = self.timings.get(key, 0.) + dt
+        self.timings.setdefault(key + '_each', []).append(dt)
         return out
 
     def kernel(self, dm0=None):
