@@ -108,3 +108,30 @@ def test_alkane_tzvp_jk_matches_oracle():
     rj, rk = O.get_jk(mol, dm, hermi=1)
     assert abs(vj - rj).max() < 1e-10
     assert abs(vk - rk).max() < 1e-10
+
+
+def test_boundary_j_only_k_only_torch_and_batched():
+    '''get_j / get_k (J-only and K-only launches), torch in -> torch out, and
+    a batch of two symmetric densities with hermi=1, against the oracle.'''
+    import torch
+    from gpu4pyscf_amd.scf import jk
+    mol, dm, _, _, _, _ = case('tzvp_hermi1')
+    opt = jk._VHFOpt(mol).build()
+    rj, rk = O.get_jk(mol, dm, hermi=1)
+    vj = jk.get_j(mol, dm, hermi=1, vhfopt=opt)
+    vk = jk.get_k(mol, dm, hermi=1, vhfopt=opt)
+    assert abs(vj - rj).max() < 1e-10
+    assert abs(vk - rk).max() < 1e-10
+    vj_t, vk_t = jk.get_jk(mol, torch.as_tensor(dm, device='cuda'), hermi=1, vhfopt=opt)
+    assert isinstance(vj_t, torch.Tensor) and vj_t.device.type == 'cuda'
+    assert abs(vj_t.cpu().numpy() - rj).max() < 1e-10
+    assert abs(vk_t.cpu().numpy() - rk).max() < 1e-10
+    rng = np.random.default_rng(6)
+    g = rng.standard_normal(dm.shape)
+    dm2 = np.stack([dm, .1 * (g + g.T)])
+    vj2, vk2 = jk.get_jk(mol, dm2, hermi=1, vhfopt=opt)
+    assert vj2.shape == dm2.shape
+    for n in range(2):
+        rj, rk = O.get_jk(mol, dm2[n], hermi=1)
+        assert abs(vj2[n] - rj).max() < 1e-10
+        assert abs(vk2[n] - rk).max() < 1e-10
