@@ -76,6 +76,9 @@ def group_by_split_points(q, split_points):
 
 
 LANE_MAX = 170      # LANE_MAX of csrc/jk_kernels.hip
+# class launches run on this many streams (32-water def2-TZVP build, ms:
+# 1 -> 648, 2 -> 569, 3 -> 564, 4 -> 566)
+NSTREAMS = 3
 
 
 def class_kernel_name(ls):
@@ -107,6 +110,11 @@ class _VHFOpt:
         # collects them
         self.profile = False
         self._profile_pending = []
+        # class tasks only add into vj/vk with atomics, so they may run
+        # concurrently: they are spread round-robin over nstreams streams
+        # (default priority) that join the caller's stream at the end
+        self.nstreams = NSTREAMS
+        self._streams = {}
 
     def profile_results(self):
         '''Per class task: (summed kernel milliseconds, surviving quartets,
@@ -314,7 +322,20 @@ class _VHFOpt:
         goff = mol.group_offsets
         g_ao0 = mol.ao_loc[goff[:-1]]
         g_nao = mol.ao_loc[goff[1:]] - g_ao0
+        cur = torch.cuda.current_stream(dms.device)
+        streams = [cur]
+        if self.nstreams > 1:
+            key = str(dms.device)
+            if key not in self._streams:
+                self._streams[key] = [torch.cuda.Stream(dms.device)
+                                      for _ in range(self.nstreams - 1)]
+            streams += self._streams[key][:self.nstreams - 1]
+            ready = torch.cuda.Event()
+            ready.record(cur)
+            for s in streams[1:]:
+                s.wait_event(ready)
         for t, (i, j, k, l) in enumerate(tasks):
+            task_stream = streams[t % len(streams)]
             args.k_ao0, args.k_nao = int(g_ao0[k]), int(g_nao[k])
             args.l_ao0, args.l_nao = int(g_ao0[l]), int(g_nao[l])
             if counter is not None:
@@ -335,14 +356,18 @@ class _VHFOpt:
             if time_classes or prof:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
-                e0.record()
-            st = lib.jk_build(ctypes.byref(args), stream)
+                e0.record(task_stream)
+            st = lib.jk_build(ctypes.byref(args), ctypes.c_void_p(task_stream.cuda_stream))
             if time_classes or prof:
-                e1.record()
+                e1.record(task_stream)
                 events.append(((i, j, k, l), e0, e1))
             if st != 0:
                 clib.check(st, f'jk_build ({"spdfg"[args.li]}{"spdfg"[args.lj]}|'
                                f'{"spdfg"[args.lk]}{"spdfg"[args.ll]})')
+        for s in streams[1:]:
+            done = torch.cuda.Event()
+            done.record(s)
+            cur.wait_event(done)
         if prof:
             self._profile_pending.append((tasks, [(e0, e1) for _, e0, e1 in events], counter))
         elif counter is not None:
