@@ -204,21 +204,37 @@ def main():
     opt.profile = False
     log(f'[bench] get_jk {dt*1e3:.1f} ms/step')
     dms_c = opt.apply_coeff_C_mat_CT(ddm[None])
+    # one more build (untimed) with every class launch on one stream: the
+    # per-launch durations of a kernel that has the GPU to itself
+    nstreams, opt.nstreams, opt.profile = opt.nstreams, 1, True
+    step()
+    torch.cuda.synchronize()
+    prof_serial = opt.profile_results()
+    opt.nstreams, opt.profile = nstreams, False
 
     uniq = opt.sorted_mol.uniq_l_ctr
+
+    def per_kernel(prof):
+        '''kernel template -> [ms, model flops, launches, quartets]; total flops'''
+        total, by = 0., {}
+        for t, (ms, cnt, nl) in prof.items():
+            ls = tuple(int(uniq[x, 0]) for x in t)
+            ps = tuple(int(uniq[x, 1]) for x in t)
+            f = cnt * quartet_flops(ls, ps, 1)
+            total += f
+            a = by.setdefault(jk.class_kernel_name(ls), [0., 0., 0, 0])
+            a[0] += ms; a[1] += f; a[2] += nl; a[3] += cnt
+        return by, total
+
     nq = sum(v[1] for v in prof.values()) // args.steps
     kern_ms = sum(v[0] for v in prof.values()) / args.steps
-    total_flops = 0.
-    by_kernel = {}
-    for t, (ms, cnt, nl) in prof.items():
-        ls = tuple(int(uniq[x, 0]) for x in t)
-        ps = tuple(int(uniq[x, 1]) for x in t)
-        f = cnt * quartet_flops(ls, ps, 1)
-        total_flops += f
-        a = by_kernel.setdefault(jk.class_kernel_name(ls), [0., 0., 0, 0])
-        a[0] += ms; a[1] += f; a[2] += nl; a[3] += cnt
+    by_kernel, total_flops = per_kernel(prof)
     dom = max(by_kernel, key=lambda k: by_kernel[k][0])
     dom_ms, dom_flops, dom_n, dom_q = by_kernel[dom]
+    by_serial, _ = per_kernel(prof_serial)
+    dom_s = max(by_serial, key=lambda k: by_serial[k][0])
+    s_ms, s_flops, s_n, s_q = by_serial[dom_s]
+    serial_ms = sum(v[0] for v in prof_serial.values())
     peak = 78.6
     tflops = lambda f, ms: f / max(ms, 1e-12) / 1e9
     log(f'[bench] quartets={nq} kernels {kern_ms:.1f} ms/step, '
@@ -228,6 +244,8 @@ def main():
         v = by_kernel[key]
         log(f'    {key}: {v[0] / args.steps:8.2f} ms/step  {tflops(v[1], v[0]):.2f} TFLOP/s  '
             f'avg launch {v[0] / v[2] * 1e3:.1f} us  launches/step {v[2] // args.steps}')
+    log(f'[bench] one-stream build: kernels {serial_ms:.1f} ms; dominant kernel {dom_s}: '
+        f'{s_ms:.1f} ms  {tflops(s_flops, s_ms):.2f} TFLOP/s  avg launch {s_ms / s_n * 1e3:.1f} us')
     # the same incremental build with FP32 quartets below fp32_tol
     vj64, vk64 = opt.get_jk(dms_c, 1)
     opt.get_jk(dms_c, 1, fp32_tol=args.fp32_tol)
@@ -257,6 +275,7 @@ def main():
             f'{e0.elapsed_time(e1):.3f} ms ({2 * nbytes / e0.elapsed_time(e1) / 1e9:.2f} TB/s)')
         del src, dst
     traffic, traffic_src = measured_traffic(args.config, dom)
+    traffic_s, _ = measured_traffic(args.config, dom_s)
     # compute bound: FP64 FMA, whose vector and matrix peaks coincide (78.6 TFLOP/s)
     roofline = dict(bound='mfma',
                     kernel=f'{dom} (all launches of the dominant kernel template)',
@@ -274,7 +293,14 @@ def main():
                                      frac=round(tflops(total_flops, kern_ms * args.steps) / peak, 4),
                                      kernel_ms_per_step=round(kern_ms, 3),
                                      achieved_wall=round(tflops(total_flops, dt * 1e3 * args.steps), 3),
-                                     frac_wall=round(tflops(total_flops, dt * 1e3 * args.steps) / peak, 4)))
+                                     frac_wall=round(tflops(total_flops, dt * 1e3 * args.steps) / peak, 4)),
+                    one_stream=dict(kernel=dom_s, avg_launch_us=round(s_ms / s_n * 1e3, 2),
+                                    achieved=round(tflops(s_flops, s_ms), 3),
+                                    frac=round(tflops(s_flops, s_ms) / peak, 4),
+                                    traffic=traffic_s, flops_per_launch=s_flops / s_n,
+                                    kernel_ms_per_build=round(serial_ms, 3),
+                                    note='one untimed build after the timed region with every '
+                                         'class launch on one stream'))
     out = dict(metric='J/K surviving shell quartets per second (one incremental get_jk, FP64)',
                value=nq / dt, unit='quartets/s', n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=dt * 1e3, higher_is_better=True,
