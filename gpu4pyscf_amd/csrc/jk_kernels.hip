@@ -448,6 +448,16 @@ __device__ __forceinline__ bool survives(float q_kl, float kl_cutoff, float dm_p
            dm_cond[jsh*nbas+lsh] > d_cutoff;
 }
 
+// SplitMix64 finaliser (Steele, Lea and Flood, OOPSLA 2014; constants of
+// Vigna's splitmix64.c): the per-quartet term of the survivor set checksum
+__device__ __forceinline__ unsigned long long quartet_mix(uint32_t bas_ij, uint32_t bas_kl)
+{
+    unsigned long long z = ((unsigned long long)bas_ij << 32) | bas_kl;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
 // Same predicate, also returning the largest of the six dm_cond values
 // (any(v > d_cutoff) == max(v) > d_cutoff).
 __device__ __forceinline__ bool survives_dmax(float q_kl, float kl_cutoff, float dm_penalty,
@@ -923,7 +933,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
         __syncthreads();
     }
     int qn64 = 0, qn32 = 0;
-    unsigned long long nsurv = 0, nsurv32 = 0;
+    unsigned long long nsurv = 0, nsurv32 = 0, hsum = 0;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     bool stop = false;
     for (int base = wi.kl0 + wid * WAVE; base < wi.kl1 || qn64 > 0 || qn32 > 0;
@@ -935,9 +945,11 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                 float q_kl = a.q_kl[pkl];
                 fail_margin = q_kl + a.dm_penalty + Q_COND_MARGIN < kl_cutoff;
                 float dmax;
-                keep = survives_dmax(q_kl, kl_cutoff, a.dm_penalty, bas_ij, a.pairs_kl[pkl],
+                const uint32_t bkl = a.pairs_kl[pkl];
+                keep = survives_dmax(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
                                      sp.ish, sp.jsh, nbas, a.dm_cond, dmax);
                 lowp = keep && (q_ij + q_kl + dmax < a.fp32_cut);
+                if (a.quartet_hash && keep) hsum += quartet_mix(bas_ij, bkl);
             }
             const uint64_t m64 = __ballot(keep && !lowp);
             const uint64_t m32 = __ballot(lowp);
@@ -983,6 +995,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
         if (a.n_quartets && nsurv) atomicAdd(a.n_quartets, nsurv);
         if (a.n_quartets_fp32 && nsurv32) atomicAdd(a.n_quartets_fp32, nsurv32);
     }
+    if (a.quartet_hash && hsum) atomicAdd(a.quartet_hash, hsum);
 }
 
 // ---------------------------------------------------------------- team
@@ -1066,7 +1079,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     double *dsh = dstage + (team < NQ ? team : 0) * S::DSZ;
     __syncthreads();
     int qn = 0;
-    unsigned long long nsurv = 0;
+    unsigned long long nsurv = 0, hsum = 0;
     bool stop = false;
     for (int base = wi.kl0; ; base += TB) {
         if (base < wi.kl1 && !stop) {
@@ -1079,6 +1092,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                 fail_margin = q_kl + a.dm_penalty + Q_COND_MARGIN < kl_cutoff;
                 keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
                                 sp.ish, sp.jsh, nbas, a.dm_cond);
+                if (a.quartet_hash && keep) hsum += quartet_mix(bas_ij, bkl);
             }
             uint64_t m = __ballot(keep);
             uint64_t mf = __ballot(fail_margin);
@@ -1340,6 +1354,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
         krows_flush(kr, NFI, NFJ, sp.i0, sp.j0, a, tid, TB);
     }
     if (a.n_quartets && tid == 0 && nsurv) atomicAdd(a.n_quartets, nsurv);
+    if (a.quartet_hash && hsum) atomicAdd(a.quartet_hash, hsum);
 }
 
 // ---------------------------------------------------------------- dispatch

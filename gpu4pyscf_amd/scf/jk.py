@@ -259,14 +259,18 @@ class _VHFOpt:
         return out
 
     def get_jk(self, dms, hermi, verbose=None, with_j=True, with_k=True,
-               count_quartets=False, time_classes=False, fp32_tol=None):
+               count_quartets=False, time_classes=False, fp32_tol=None,
+               hash_quartets=False):
         '''J/K for density matrices ``dms`` (n_dm, nao_c, nao_c) in the sorted
         Cartesian basis (torch, float64, on the device).
 
         count_quartets: store the surviving quartets of each class task in
         ``self.last_task_counts`` (and their sum in ``self.last_quartets``).
+        hash_quartets: also store the set checksum of each task's surviving
+        quartets (sum mod 2**64 of the SplitMix64 finaliser of
+        bas_ij << 32 | bas_kl) in ``self.last_task_hashes``.
         time_classes: store (task, milliseconds) of each class launch, timed
-        with HIP events on the current stream, in ``self.last_task_ms``.
+        with HIP events on the stream it runs on, in ``self.last_task_ms``.
         fp32_tol: evaluate in FP32 the quartets whose bound
         sqrt|(ij|ij)| sqrt|(kl|kl)| max|D| is below fp32_tol (None: all FP64).'''
         torch = _torch()
@@ -300,6 +304,8 @@ class _VHFOpt:
                    if count_quartets else None)
         counter32 = (torch.zeros(len(tasks), dtype=torch.int64, device=dms.device)
                      if count_quartets else None)
+        hashes = (torch.zeros(len(tasks), dtype=torch.int64, device=dms.device)
+                  if hash_quartets else None)
         events = []
 
         args = clib.JKBuildArgs()
@@ -341,6 +347,8 @@ class _VHFOpt:
             if counter is not None:
                 args.n_quartets = counter.data_ptr() + 8 * t
                 args.n_quartets_fp32 = counter32.data_ptr() + 8 * t
+            if hashes is not None:
+                args.quartet_hash = hashes.data_ptr() + 8 * t
             pij, qij = d['pairs'][i, j], d['q'][i, j]
             pkl, qkl = d['pairs'][k, l], d['q'][k, l]
             args.li, args.lj, args.lk, args.ll = (int(uniq[x, 0]) for x in (i, j, k, l))
@@ -374,6 +382,9 @@ class _VHFOpt:
             self.last_task_counts = dict(zip(tasks, counter.cpu().numpy().tolist()))
             self.last_quartets = int(sum(self.last_task_counts.values()))
             self.last_quartets_fp32 = int(counter32.sum().item())
+        if hashes is not None:
+            self.last_task_hashes = dict(zip(tasks, (int(h) & (2**64 - 1)
+                                                     for h in hashes.cpu().numpy().tolist())))
         if time_classes and not prof:
             torch.cuda.synchronize()
             self.last_task_ms = [(t, e0.elapsed_time(e1)) for t, e0, e1 in events]

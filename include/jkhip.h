@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  6
+#define JK_ABI_VERSION  7
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         3       /* max angular momentum on the GPU path */
 
@@ -83,6 +83,14 @@ typedef struct {
      * counterpart in the reference. */
     float fp32_cut;
     unsigned long long *n_quartets_fp32;    /* device counter or NULL */
+    /* Set checksum of the surviving quartets, for set-equality tests: adds,
+     * mod 2^64, jk_quartet_mix(bas_ij << 32 | bas_kl) of every surviving
+     * quartet, where jk_quartet_mix is the SplitMix64 finaliser
+     *   z = (z ^ z >> 30) * 0xbf58476d1ce4e5b9;
+     *   z = (z ^ z >> 27) * 0x94d049bb133111eb;  return z ^ z >> 31.
+     * Device counter or NULL.  No counterpart in the reference, whose
+     * survivor queue (create_tasks.cu:114-180) is internal to the kernel. */
+    unsigned long long *quartet_hash;
 } JKBuildArgs;
 
 /* Accumulate one class task into vj/vk (before the hermi post-processing of

@@ -41,6 +41,8 @@ def lib():
             subprocess.check_call(['make', '-s', '-C', _here])
         _lib = ctypes.CDLL(path)
         _lib.oracle_jk.restype = ctypes.c_long
+        _lib.oracle_splitmix64_mix.restype = ctypes.c_uint64
+        _lib.oracle_splitmix64_mix.argtypes = [ctypes.c_uint64]
     return _lib
 
 
@@ -267,6 +269,30 @@ def survivors(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13):
     nb2 = np.uint64(sorted_mol.nbas) ** 2
     keys = out[:, 0].astype(np.uint64) * nb2 + out[:, 1].astype(np.uint64)
     return np.sort(keys)
+
+
+def quartet_sets(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13):
+    '''{(i, j, k, l): (count, checksum)} of the surviving quartets of every
+    class task with survivors: checksum = sum mod 2**64 of the SplitMix64
+    finaliser of bas_ij << 32 | bas_kl (the C oracle's quartet_mix).'''
+    dms = np.asarray(dms, dtype=np.float64)
+    cond, penalty = dm_condition(dms, sorted_mol.ao_loc, hermi)
+    ng = len(sorted_mol.l_ctr_counts)
+    off, pairs, q = flatten_pairs(cache, ng)
+    tasks = [(i, j, k, l) for i in range(ng) for j in range(i + 1)
+             for k in range(i + 1) for l in range(k + 1)]
+    counts = np.zeros(len(tasks), dtype=np.int64)
+    hashes = np.zeros(len(tasks), dtype=np.uint64)
+    lib().oracle_quartet_sets(
+        ctypes.c_int(sorted_mol.nbas), ctypes.c_int(ng), _p(off), _p(pairs), _p(q),
+        _p(cond), ctypes.c_float(np.float32(math.log(direct_scf_tol))),
+        ctypes.c_float(penalty), _p(counts), _p(hashes))
+    return {t: (int(c), int(h)) for t, c, h in zip(tasks, counts, hashes) if c}
+
+
+def splitmix64_mix(z):
+    '''The C oracle's SplitMix64 finaliser of a 64-bit integer.'''
+    return int(lib().oracle_splitmix64_mix(ctypes.c_uint64(z)))
 
 
 def get_jk(mol, dm, hermi=1, with_j=True, with_k=True, direct_scf_tol=1e-13,

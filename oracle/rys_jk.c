@@ -458,3 +458,58 @@ long oracle_jk(int n_dm, int nao, const double *dms, double *vj, double *vk,
     free(tasks);
     return total;
 }
+
+/* SplitMix64 finaliser (Steele, Lea and Flood, OOPSLA 2014; constants of
+ * Vigna's splitmix64.c), the per-quartet term of a survivor set checksum. */
+static inline uint64_t quartet_mix(uint32_t bij, uint32_t bkl)
+{
+    uint64_t z = ((uint64_t)bij << 32) | bkl;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+uint64_t oracle_splitmix64_mix(uint64_t z)
+{
+    return quartet_mix((uint32_t)(z >> 32), (uint32_t)z);
+}
+
+/*
+ * Surviving quartets of every task (same enumeration and predicate as
+ * oracle_jk, no integrals): counts[t] = number of survivors and
+ * hashes[t] = sum mod 2^64 of quartet_mix(bas_ij, bas_kl) over them, for
+ * t over all (i>=j, k<=i, l<=k) group quadruples in oracle_jk's order.  Every
+ * kl candidate is tested (no early exit on the sorted bins).
+ */
+void oracle_quartet_sets(int nbas, int ngroups, const int *pair_off,
+                         const uint32_t *pairs, const float *q_cond,
+                         const float *dm_cond, float cutoff, float dm_penalty,
+                         int64_t *counts, uint64_t *hashes)
+{
+    int t = 0;
+    for (int i = 0; i < ngroups; i++)
+    for (int j = 0; j <= i; j++)
+    for (int k = 0; k <= i; k++)
+    for (int l = 0; l <= k; l++, t++) {
+        int pij0 = pair_off[i*(i+1)/2+j], pij1 = pair_off[i*(i+1)/2+j+1];
+        int pkl0 = pair_off[k*(k+1)/2+l], pkl1 = pair_off[k*(k+1)/2+l+1];
+        int64_t n = 0;
+        uint64_t h = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+:n, h)
+        for (int pij = pij0; pij < pij1; pij++) {
+            uint32_t bij = pairs[pij];
+            int ish = bij / nbas, jsh = bij % nbas;
+            for (int pkl = pkl0; pkl < pkl1; pkl++) {
+                uint32_t bkl = pairs[pkl];
+                int ksh = bkl / nbas, lsh = bkl % nbas;
+                if (survives(q_cond[pij], q_cond[pkl], cutoff, dm_penalty,
+                             bij, bkl, ish, jsh, ksh, lsh, nbas, dm_cond)) {
+                    n++;
+                    h += quartet_mix(bij, bkl);
+                }
+            }
+        }
+        counts[t] = n;
+        hashes[t] = h;
+    }
+}

@@ -69,6 +69,57 @@ def test_water8_svp_jk_and_quartet_set():
     assert abs(vk.cpu().numpy() - rk).max() < 1e-10
 
 
+def _gpu_task_sets(opt):
+    return {t: (c, opt.last_task_hashes[t])
+            for t, c in opt.last_task_counts.items() if c}
+
+
+def test_water8_svp_quartet_set_checksums():
+    '''Per class task, the surviving-quartet count and set checksum of the
+    kernels equal the oracle's scan of the restated predicate.'''
+    import torch
+    from gpu4pyscf_amd.scf import jk
+    mol, dm = _water8_svp()
+    opt = jk._VHFOpt(mol).build()
+    dms = opt.apply_coeff_C_mat_CT(torch.as_tensor(dm, device='cuda')[None])
+    opt.get_jk(dms, 1, count_quartets=True, hash_quartets=True)
+    cache = {k: (v[0], v[1]) for k, v in opt.bas_pair_cache.items()}
+    ref = O.quartet_sets(opt.sorted_mol, dms.cpu().numpy(), 1, cache, opt.direct_scf_tol)
+    assert _gpu_task_sets(opt) == ref
+
+
+def test_water32_tzvp_full_size_quartet_set():
+    '''The bench workload at full size (32 waters, def2-TZVP, 1376 AOs, one
+    incremental build on dD = 1e-3 (G + G^T)/2): screening pair lists equal
+    the oracle's, and per class task the surviving quartets (count and set
+    checksum, 1.39e9 quartets) equal the oracle's full scan, which has no
+    early exit, in FP64 and in mixed precision.'''
+    import torch
+    from gpu4pyscf_amd.gto import geometry as G
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.scf import jk
+    mol = M(atom=G.to_string(G.water_cluster((4, 4, 2), 3.0, seed=0)), basis='def2-tzvp')
+    assert mol.nao == 1376
+    print('[water32 def2-TZVP] screening setup', flush=True)
+    opt = jk._VHFOpt(mol).build()
+    ref_cache = O.build_pairs(opt.sorted_mol, opt.direct_scf_tol)
+    for key, (p, q, _) in opt.bas_pair_cache.items():
+        rp, rq = ref_cache[key]
+        assert np.array_equal(p, rp), key
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal((mol.nao, mol.nao))
+    ddm = torch.as_tensor(1e-3 * (g + g.T) / 2, device='cuda')
+    dms = opt.apply_coeff_C_mat_CT(ddm[None])
+    cache = {k: (v[0], v[1]) for k, v in opt.bas_pair_cache.items()}
+    print('[water32 def2-TZVP] oracle scan', flush=True)
+    ref = O.quartet_sets(opt.sorted_mol, dms.cpu().numpy(), 1, cache, opt.direct_scf_tol)
+    assert sum(c for c, _ in ref.values()) == 1389051040
+    for tol in (None, 1e-6):
+        print(f'[water32 def2-TZVP] get_jk fp32_tol={tol}', flush=True)
+        opt.get_jk(dms, 1, count_quartets=True, hash_quartets=True, fp32_tol=tol)
+        assert _gpu_task_sets(opt) == ref
+
+
 def test_mixed_precision_incremental_jk():
     '''FP32 quartets on an incremental density change: error against the
     FP64 build stays below the bound implied by the threshold, and a large

@@ -227,6 +227,51 @@ def test_screening_drops_only_negligible():
     assert len(keys) > 0
 
 
+def test_splitmix64_published_sequence():
+    '''The checksum mix is the SplitMix64 finaliser: the generator built on it
+    (state += 0x9e3779b97f4a7c15, output mix(state)) from seed 1234567 gives
+    the published sequence of Vigna's splitmix64.c.'''
+    s, out = 1234567, []
+    for _ in range(5):
+        s = (s + 0x9e3779b97f4a7c15) % 2**64
+        out.append(O.splitmix64_mix(s))
+    assert out == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                   4593380528125082431, 16408922859458223821]
+
+
+def test_quartet_set_checksums_match_survivor_list():
+    '''Per-task survivor counts and set checksums of the OpenMP scan equal
+    those recomputed from the serial survivor list (8-water def2-SVP,
+    random symmetric density).'''
+    from gpu4pyscf_amd.gto import geometry as G
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    mol = M(atom=G.to_string(G.water_cluster((2, 2, 2), 3.0, seed=0)), basis='def2-svp')
+    sm = SortedGTO(mol)
+    cache = O.build_pairs(sm, 1e-13)
+    rng = np.random.default_rng(3)
+    g = rng.standard_normal((1, sm.nao, sm.nao))
+    d = 1e-3 * (g + g.transpose(0, 2, 1))
+    sets = O.quartet_sets(sm, d, 1, cache)
+    keys = O.survivors(sm, d, 1, cache)
+    nb = sm.nbas
+    bij, bkl = keys // np.uint64(nb * nb), keys % np.uint64(nb * nb)
+    sh = np.stack([bij // np.uint64(nb), bij % np.uint64(nb),
+                   bkl // np.uint64(nb), bkl % np.uint64(nb)]).astype(np.int64)
+    grp = np.searchsorted(np.asarray(sm.group_offsets), sh, side='right') - 1
+    z = (bij << np.uint64(32)) | bkl
+    with np.errstate(over='ignore'):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+    z = z ^ (z >> np.uint64(31))
+    ref = {}
+    for t, h in zip(map(tuple, grp.T.tolist()), z.tolist()):
+        c, hh = ref.get(t, (0, 0))
+        ref[t] = (c + 1, (hh + h) % 2**64)
+    assert len(keys) > 10**6
+    assert sets == ref
+
+
 def test_flops_model():
     from gpu4pyscf_amd.scf.flops import quartet_flops, quartet_digest_bytes
     # (ss|ss), one primitive: 1 root, 3 + 9 flops + 12 digestion
