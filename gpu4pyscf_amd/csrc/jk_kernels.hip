@@ -124,6 +124,17 @@ __device__ __forceinline__ void atomic_add(double *p, double v)
     unsafeAtomicAdd(p, v);
 }
 
+// Replicated accumulation (JKBuildArgs.n_repl): this workgroup's copy of
+// vj/vk.
+__device__ __forceinline__ void select_replica(JKBuildArgs &a)
+{
+    if (a.n_repl > 1) {
+        const int64_t off = (int64_t)(blockIdx.x % a.n_repl) * a.repl_stride;
+        if (a.vj) a.vj += off;
+        if (a.vk) a.vk += off;
+    }
+}
+
 // LDS accumulation: workgroup scope, so that it cannot be merged with the
 // device-scope HBM atomics of a sibling branch into one generic atomic
 __device__ __forceinline__ void lds_add(double *p, double v)
@@ -917,6 +928,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     uint32_t *queue64 = queue_all[wid][0];
     uint32_t *queue32 = queue_all[wid][1];
     const int nbas = a.nbas, nao = a.nao;
+    select_replica(a);
     WorkItem wi = work_item(a.npairs_kl, kl_chunk);
     const uint32_t bas_ij = a.pairs_ij[wi.pij];
     const float q_ij = a.q_ij[wi.pij];
@@ -1055,6 +1067,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     const int tl = tid - team * TS;
     const int nbas = a.nbas, nao = a.nao;
     const size_t nao2 = (size_t)nao * nao;
+    select_replica(a);
     WorkItem wi = work_item(a.npairs_kl, kl_chunk);
     const uint32_t bas_ij = a.pairs_ij[wi.pij];
     const float kl_cutoff = a.cutoff - a.q_ij[wi.pij];
@@ -1680,6 +1693,10 @@ int jk_build(const JKBuildArgs *args, void *stream)
         return 3;
     }
     if (!table_ok(a.rys)) { set_err("jk_build: Rys table not set"); return 3; }
+    if (a.n_repl > 1 && a.repl_stride < (int64_t)a.n_dm * a.nao * a.nao) {
+        set_err("jk_build: repl_stride %s%d x nao^2 too small", "", a.n_dm);
+        return 3;
+    }
     int nroots = (a.li + a.lj + a.lk + a.ll) / 2 + 1;
     if (nroots > JK_RYS_NMAX || a.rys.xmax[nroots-1] <= 0) {
         set_err("jk_build: Rys table lacks %s%d roots", "", nroots);

@@ -14,7 +14,7 @@ _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, 'libjkhip.so')
 TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
 
-JK_ABI_VERSION = 7
+JK_ABI_VERSION = 8
 JK_RYS_NMAX = 11
 JK_BOYS_M = 13
 JK_LMAX = 3
@@ -67,6 +67,8 @@ class JKBuildArgs(ctypes.Structure):
         ('fp32_cut', ctypes.c_float),
         ('n_quartets_fp32', c_void_p),
         ('quartet_hash', c_void_p),
+        ('n_repl', ctypes.c_int),
+        ('repl_stride', ctypes.c_int64),
     ]
 
 

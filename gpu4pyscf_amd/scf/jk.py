@@ -79,6 +79,10 @@ LANE_MAX = 170      # LANE_MAX of csrc/jk_kernels.hip
 # class launches run on this many streams (32-water def2-TZVP build, ms:
 # 1 -> 648, 2 -> 569, 3 -> 564, 4 -> 566)
 NSTREAMS = 3
+# copies of vj/vk per build (32-water def2-TZVP build, ms, one stream:
+# 1 -> 653, 2 -> 639, 4 -> 630, 8 -> 627, 16 -> 626; three streams:
+# 1 -> 562, 8 -> 549)
+NREPL = 8
 
 
 def class_kernel_name(ls):
@@ -115,6 +119,8 @@ class _VHFOpt:
         # (default priority) that join the caller's stream at the end
         self.nstreams = NSTREAMS
         self._streams = {}
+        # copies of vj/vk the class kernels add into (one per XCD)
+        self.n_repl = NREPL
 
     def profile_results(self):
         '''Per class task: (summed kernel milliseconds, surviving quartets,
@@ -295,8 +301,12 @@ class _VHFOpt:
         if hermi == 0:
             dms = torch.cat([dms, dms.transpose(1, 2)]).contiguous()
         n_dm = dms.shape[0]
-        vj = torch.zeros_like(dms) if with_j else None
-        vk = torch.zeros_like(dms) if with_k else None
+        # n_repl zeroed copies of vj/vk; workgroup b adds into copy
+        # b % n_repl (one per XCD), the copies are summed after the build
+        n_repl = max(int(self.n_repl), 1)
+        shape = (n_repl,) + tuple(dms.shape)
+        vj = torch.zeros(shape, dtype=dms.dtype, device=dms.device) if with_j else None
+        vk = torch.zeros(shape, dtype=dms.dtype, device=dms.device) if with_k else None
         tasks = self.tasks()
         prof = self.profile
         count_quartets = count_quartets or prof
@@ -324,6 +334,8 @@ class _VHFOpt:
         args.vk = vk.data_ptr() if with_k else None
         args.rys = self._rys
         args.fp32_cut = np.float32(math.log(fp32_tol)) if fp32_tol else np.float32(-1e30)
+        args.n_repl = n_repl
+        args.repl_stride = dms.numel()
         uniq = mol.uniq_l_ctr
         goff = mol.group_offsets
         g_ao0 = mol.ao_loc[goff[:-1]]
@@ -376,6 +388,10 @@ class _VHFOpt:
             done = torch.cuda.Event()
             done.record(s)
             cur.wait_event(done)
+        if with_j:
+            vj = vj[0] if n_repl == 1 else vj.sum(0)
+        if with_k:
+            vk = vk[0] if n_repl == 1 else vk.sum(0)
         if prof:
             self._profile_pending.append((tasks, [(e0, e1) for _, e0, e1 in events], counter))
         elif counter is not None:

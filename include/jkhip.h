@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  7
+#define JK_ABI_VERSION  8
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         3       /* max angular momentum on the GPU path */
 
@@ -91,6 +91,16 @@ typedef struct {
      * Device counter or NULL.  No counterpart in the reference, whose
      * survivor queue (create_tasks.cu:114-180) is internal to the kernel. */
     unsigned long long *quartet_hash;
+    /* Replicated accumulation: with n_repl > 1, vj and vk each point to
+     * n_repl zeroed copies of the n_dm x nao x nao block, repl_stride
+     * doubles apart, and workgroup b adds into copy b % n_repl; the caller
+     * sums the copies afterwards.  Workgroups are dispatched round-robin
+     * over the 8 XCDs of an MI355X, so n_repl = 8 gives each XCD its own
+     * copy, which keeps the XCDs' atomic adds off each other's cache
+     * lines.  0 or 1: plain accumulation into vj/vk.  No counterpart in the
+     * reference. */
+    int n_repl;
+    int64_t repl_stride;
 } JKBuildArgs;
 
 /* Accumulate one class task into vj/vk (before the hermi post-processing of

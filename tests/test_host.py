@@ -51,6 +51,17 @@ def test_jk_build_rejects_bad_arguments():
     a.npairs_ij = a.npairs_kl = 3
     assert lib.jk_build(ctypes.byref(a), None) == 3
     assert b'null' in lib.jk_last_error()
+    # replicated accumulation with copies closer than one n_dm x nao^2 block
+    # (placeholder addresses: nothing is dereferenced before the check)
+    for f in ('bas', 'env', 'ao_loc', 'pairs_ij', 'pairs_kl', 'q_ij', 'q_kl', 'pdata_ij',
+              'pdata_kl', 'dm_cond', 'dm', 'vj', 'vk'):
+        setattr(a, f, 4096)
+    a.nbas, a.nao, a.n_dm = 4, 10, 1
+    a.rys.coef = 4096
+    a.rys.degree = 11
+    a.n_repl, a.repl_stride = 8, 99
+    assert lib.jk_build(ctypes.byref(a), None) == 3
+    assert b'repl_stride' in lib.jk_last_error()
 
 
 def test_tile_pairs_and_bins_match_oracle():
