@@ -917,8 +917,25 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
 // rows); each wavefront scans every LANE_WAVES-th batch of 64 kl candidates
 // and keeps two queues: quartets whose bound q_ij + q_kl + max(dm_cond)
 // reaches fp32_cut run in FP64, the others in FP32.
+// Minimum waves per SIMD a class kernel is compiled for.  By default the
+// compiler may use all 512 registers (one wave per SIMD); the classes below
+// need a little over 256 and run faster with two waves per SIMD despite a few
+// spilled registers (32-water def2-TZVP build, ms, one wave -> two:
+// (fs|ps) 23.1 -> 16.9, (ds|pp) 15.2 -> 13.2, (ds|ds) 9.9 -> 7.1,
+// (dd|pp) 11.6 -> 8.8, (fp|fp) 11.2 -> 8.5, (fd|dp) 9.4 -> 6.2; the classes
+// that need 350 or more registers ran slower with two waves).
+template <int LI, int LJ, int LK, int LL> struct ClassOcc { static constexpr int W = 1; };
+#define CLASS_OCC2(a, b, c, d) \
+    template <> struct ClassOcc<a, b, c, d> { static constexpr int W = 2; };
+CLASS_OCC2(3, 0, 1, 0) CLASS_OCC2(2, 0, 1, 1) CLASS_OCC2(2, 0, 2, 0)
+CLASS_OCC2(3, 1, 3, 1) CLASS_OCC2(3, 2, 2, 1) CLASS_OCC2(2, 2, 1, 1)
+CLASS_OCC2(2, 1, 2, 2) CLASS_OCC2(3, 3, 3, 0) CLASS_OCC2(2, 2, 2, 1)
+CLASS_OCC2(3, 0, 3, 2) CLASS_OCC2(3, 1, 2, 2) CLASS_OCC2(3, 3, 2, 1)
+CLASS_OCC2(3, 2, 3, 0)
+#undef CLASS_OCC2
+
 template <int LI, int LJ, int LK, int LL>
-__global__ __launch_bounds__(LANE_WAVES * WAVE)
+__global__ __launch_bounds__(LANE_WAVES * WAVE, (ClassOcc<LI, LJ, LK, LL>::W))
 void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
 {
     constexpr int NFI = ncart(LI), NFJ = ncart(LJ);
@@ -1039,7 +1056,9 @@ struct TeamShape {
 };
 
 template <int LI, int LJ, int LK, int LL>
-__global__ __launch_bounds__((TeamShape<LI, LJ, LK, LL>::TB), TEAM_MIN_WAVES)
+__global__ __launch_bounds__((TeamShape<LI, LJ, LK, LL>::TB),
+                             (TEAM_MIN_WAVES > ClassOcc<LI, LJ, LK, LL>::W ?
+                              TEAM_MIN_WAVES : ClassOcc<LI, LJ, LK, LL>::W))
 void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
 {
     using S = TeamShape<LI, LJ, LK, LL>;
