@@ -352,8 +352,12 @@ class _VHFOpt:
             ready.record(cur)
             for s in streams[1:]:
                 s.wait_event(ready)
-        for t, (i, j, k, l) in enumerate(tasks):
-            task_stream = streams[t % len(streams)]
+        # launched from the last task (f classes, the longest) to the first,
+        # so that the short s/p classes fill the streams at the end
+        # (32-water def2-TZVP build: 521 ms in task order, 515 reversed)
+        for n, t in enumerate(range(len(tasks) - 1, -1, -1)):
+            i, j, k, l = tasks[t]
+            task_stream = streams[n % len(streams)]
             args.k_ao0, args.k_nao = int(g_ao0[k]), int(g_nao[k])
             args.l_ao0, args.l_nao = int(g_ao0[l]), int(g_nao[l])
             if counter is not None:
@@ -380,7 +384,7 @@ class _VHFOpt:
             st = lib.jk_build(ctypes.byref(args), ctypes.c_void_p(task_stream.cuda_stream))
             if time_classes or prof:
                 e1.record(task_stream)
-                events.append(((i, j, k, l), e0, e1))
+                events.append((t, e0, e1))
             if st != 0:
                 clib.check(st, f'jk_build ({"spdfg"[args.li]}{"spdfg"[args.lj]}|'
                                f'{"spdfg"[args.lk]}{"spdfg"[args.ll]})')
@@ -393,6 +397,7 @@ class _VHFOpt:
         if with_k:
             vk = vk[0] if n_repl == 1 else vk.sum(0)
         if prof:
+            events.sort(key=lambda e: e[0])
             self._profile_pending.append((tasks, [(e0, e1) for _, e0, e1 in events], counter))
         elif counter is not None:
             self.last_task_counts = dict(zip(tasks, counter.cpu().numpy().tolist()))
@@ -403,7 +408,7 @@ class _VHFOpt:
                                                      for h in hashes.cpu().numpy().tolist())))
         if time_classes and not prof:
             torch.cuda.synchronize()
-            self.last_task_ms = [(t, e0.elapsed_time(e1)) for t, e0, e1 in events]
+            self.last_task_ms = [(tasks[t], e0.elapsed_time(e1)) for t, e0, e1 in sorted(events)]
         if hermi == 1:
             if with_j:
                 vj *= 2.
