@@ -150,6 +150,44 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Sums M values over the 64 lanes of a wavefront and hands each total to
+// one lane (reduce-scatter by recursive halving): at the step with lane bit
+// b, lanes with bit b set keep the upper half of their values and add the
+// partner's upper half, the others the lower half.  About M shuffles instead
+// of 6 M for M separate butterfly sums.  v holds the M values of this lane
+// and is overwritten; `put(n, x)` is called for every n < M, on the lane
+// that holds the total x of value n.
+template <int M, int BIT, typename F>
+__device__ __forceinline__ void reduce_scatter_step(double *v, int lane, int base, int nvalid,
+                                                    F &put)
+{
+    if constexpr (BIT == 0) {
+#pragma unroll
+        for (int t = 0; t < M; ++t)
+            if (t < nvalid) put(base + t, v[t]);
+    } else {
+        constexpr int H = (M + 1) / 2;
+        const bool hi = (lane & BIT) != 0;
+        double w[H];
+#pragma unroll
+        for (int t = 0; t < H; ++t) {
+            const double lo_v = v[t];
+            const double hi_v = (H + t < M) ? v[H + t] : 0.;
+            const double mine = hi ? hi_v : lo_v;
+            const double send = hi ? lo_v : hi_v;
+            w[t] = mine + __shfl_xor(send, BIT, WAVE);
+        }
+        const int nv = hi ? (nvalid > H ? nvalid - H : 0) : (nvalid < H ? nvalid : H);
+        reduce_scatter_step<H, BIT / 2>(w, lane, hi ? base + H : base, nv, put);
+    }
+}
+
+template <int M, typename F>
+__device__ __forceinline__ void wave_reduce_scatter(double *v, int lane, F &&put)
+{
+    reduce_scatter_step<M, WAVE / 2>(v, lane, 0, M, put);
+}
+
 // Orders LDS accesses of the lanes of one wavefront (no block barrier).
 __device__ __forceinline__ void wave_sync()
 {
@@ -902,13 +940,11 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                         s = fma((double)gout[((i*NFJ + j)*NFK + k)*NFL + l],
                                 D[(k0+k)*nao + l0+l], s);
                 }
-                vij[i*NFJ + j] = wave_sum(s);
+                vij[i*NFJ + j] = s;
             }
-            if (lane == 0) {
-#pragma unroll
-                for (int n = 0; n < NFI*NFJ; ++n)
-                    atomic_add(J + (i0 + n / NFJ)*nao + j0 + n % NFJ, vij[n]);
-            }
+            wave_reduce_scatter<NFI*NFJ>(vij, lane, [&](int n, double x) {
+                atomic_add(J + (i0 + n / NFJ)*nao + j0 + n % NFJ, x);
+            });
         }
     }
 }
