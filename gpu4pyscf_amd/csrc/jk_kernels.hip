@@ -1726,6 +1726,36 @@ const char *jk_last_error(void) { return g_err; }
 
 int jk_abi_version(void) { return JK_ABI_VERSION; }
 
+int jk_build(const JKBuildArgs *args, void *stream);
+
+int jk_build_tasks(const JKBuildArgs *args, const JKTask *tasks, int ntasks,
+                   void *const *streams, int nstreams)
+{
+    g_err[0] = 0;
+    if (!args || (ntasks > 0 && !tasks) || ntasks < 0 || nstreams < 1 || !streams) {
+        set_err("jk_build_tasks: null pointer or bad count (%s%d tasks)", "", ntasks);
+        return 1;
+    }
+    JKBuildArgs a = *args;
+    for (int m = 0; m < ntasks; ++m) {
+        const int t = ntasks - 1 - m;
+        const JKTask &k = tasks[t];
+        a.li = k.li; a.lj = k.lj; a.lk = k.lk; a.ll = k.ll;
+        a.iprim = k.iprim; a.jprim = k.jprim; a.kprim = k.kprim; a.lprim = k.lprim;
+        a.k_ao0 = k.k_ao0; a.k_nao = k.k_nao; a.l_ao0 = k.l_ao0; a.l_nao = k.l_nao;
+        a.npairs_ij = k.npairs_ij; a.npairs_kl = k.npairs_kl;
+        a.pairs_ij = k.pairs_ij; a.pairs_kl = k.pairs_kl;
+        a.q_ij = k.q_ij; a.q_kl = k.q_kl;
+        a.pdata_ij = k.pdata_ij; a.pdata_kl = k.pdata_kl;
+        a.n_quartets = args->n_quartets ? args->n_quartets + t : nullptr;
+        a.n_quartets_fp32 = args->n_quartets_fp32 ? args->n_quartets_fp32 + t : nullptr;
+        a.quartet_hash = args->quartet_hash ? args->quartet_hash + t : nullptr;
+        const int st = jk_build(&a, streams[m % nstreams]);
+        if (st != 0) return st;
+    }
+    return 0;
+}
+
 int jk_build(const JKBuildArgs *args, void *stream)
 {
     g_err[0] = 0;

@@ -72,7 +72,15 @@ class JKBuildArgs(ctypes.Structure):
     ]
 
 
-EXPORTS = ('jk_build', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
+class JKTask(ctypes.Structure):
+    '''JKTask of include/jkhip.h: the per-task fields of JKBuildArgs.'''
+    _fields_ = [(n, ctypes.c_int) for n in ('li', 'lj', 'lk', 'll', 'iprim', 'jprim', 'kprim',
+                                            'lprim', 'k_ao0', 'k_nao', 'l_ao0', 'l_nao',
+                                            'npairs_ij', 'npairs_kl')] + [
+        (n, c_void_p) for n in ('pairs_ij', 'pairs_kl', 'q_ij', 'q_kl', 'pdata_ij', 'pdata_kl')]
+
+
+EXPORTS = ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
            'jk_int1e', 'mlp_gemm_bf16', 'mlp_silu_backward', 'mlp_colsum', 'mlp_mse_grad',
            'mlp_adamw', 'mlp_to_bf16', 'jk_last_error', 'jk_abi_version')
 
@@ -89,7 +97,7 @@ def load():
                            'or make -C gpu4pyscf_amd/csrc')
     lib = ctypes.CDLL(LIB_PATH)
     lib.jk_last_error.restype = ctypes.c_char_p
-    for name in ('jk_build', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
+    for name in ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
                  'jk_abi_version'):
         getattr(lib, name).restype = ctypes.c_int
     lib.jk_build.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p]

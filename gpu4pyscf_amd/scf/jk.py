@@ -225,6 +225,7 @@ class _VHFOpt:
         d['pairs'] = dev_pairs
         d['q'] = dev_q
         d['pdata'] = dev_pd
+        d['task_array'] = self._task_array(d)
         self._dev = d
         self.device = dev
         self.timings['build'] = time.perf_counter() - t0
@@ -263,6 +264,32 @@ class _VHFOpt:
                         if len(self.bas_pair_cache[i, j][0]) and len(self.bas_pair_cache[k, l][0]):
                             out.append((i, j, k, l))
         return out
+
+    def _task_array(self, d):
+        '''ctypes array of JKTask (include/jkhip.h) for self.tasks(), built
+        once per geometry: the per-task launch arguments of jk_build_tasks.'''
+        mol = self.sorted_mol
+        uniq = mol.uniq_l_ctr
+        goff = mol.group_offsets
+        g_ao0 = mol.ao_loc[goff[:-1]]
+        g_nao = mol.ao_loc[goff[1:]] - g_ao0
+        tasks = self.tasks()
+        arr = (clib.JKTask * max(len(tasks), 1))()
+        for t, (i, j, k, l) in enumerate(tasks):
+            a = arr[t]
+            a.li, a.lj, a.lk, a.ll = (int(uniq[x, 0]) for x in (i, j, k, l))
+            a.iprim, a.jprim, a.kprim, a.lprim = (int(uniq[x, 1]) for x in (i, j, k, l))
+            a.k_ao0, a.k_nao = int(g_ao0[k]), int(g_nao[k])
+            a.l_ao0, a.l_nao = int(g_ao0[l]), int(g_nao[l])
+            a.npairs_ij = len(d['pairs'][i, j])
+            a.npairs_kl = len(d['pairs'][k, l])
+            a.pairs_ij = d['pairs'][i, j].data_ptr()
+            a.pairs_kl = d['pairs'][k, l].data_ptr()
+            a.q_ij = d['q'][i, j].data_ptr()
+            a.q_kl = d['q'][k, l].data_ptr()
+            a.pdata_ij = d['pdata'][i, j].data_ptr()
+            a.pdata_kl = d['pdata'][k, l].data_ptr()
+        return arr
 
     def get_jk(self, dms, hermi, verbose=None, with_j=True, with_k=True,
                count_quartets=False, time_classes=False, fp32_tol=None,
@@ -354,8 +381,21 @@ class _VHFOpt:
                 s.wait_event(ready)
         # launched from the last task (f classes, the longest) to the first,
         # so that the short s/p classes fill the streams at the end
-        # (32-water def2-TZVP build: 521 ms in task order, 515 reversed)
-        for n, t in enumerate(range(len(tasks) - 1, -1, -1)):
+        # (32-water def2-TZVP build: 521 ms in task order, 515 reversed).
+        # Without per-launch timing the whole loop is one native call.
+        if not (time_classes or prof):
+            if counter is not None:
+                args.n_quartets = counter.data_ptr()
+                args.n_quartets_fp32 = counter32.data_ptr()
+            if hashes is not None:
+                args.quartet_hash = hashes.data_ptr()
+            handles = (ctypes.c_void_p * len(streams))(*[s.cuda_stream for s in streams])
+            clib.check(lib.jk_build_tasks(ctypes.byref(args), d['task_array'], len(tasks),
+                                          handles, len(streams)), 'jk_build_tasks')
+            tasks_looped = []
+        else:
+            tasks_looped = range(len(tasks) - 1, -1, -1)
+        for n, t in enumerate(tasks_looped):
             i, j, k, l = tasks[t]
             task_stream = streams[n % len(streams)]
             args.k_ao0, args.k_nao = int(g_ao0[k]), int(g_nao[k])

@@ -107,6 +107,28 @@ typedef struct {
  * gpu4pyscf/scf/jk.py:594-616).  stream: hipStream_t or NULL. */
 int jk_build(const JKBuildArgs *args, void *stream);
 
+/* The per-task fields of JKBuildArgs, for a whole build in one call. */
+typedef struct {
+    int li, lj, lk, ll;
+    int iprim, jprim, kprim, lprim;
+    int k_ao0, k_nao, l_ao0, l_nao;
+    int npairs_ij, npairs_kl;
+    const uint32_t *pairs_ij, *pairs_kl;
+    const float *q_ij, *q_kl;
+    const double *pdata_ij, *pdata_kl;
+} JKTask;
+
+/* All class tasks of one build: the task loop of gpu4pyscf/scf/jk.py:554-585
+ * in one call.  *args supplies the fields shared by the tasks; tasks[t] the
+ * per-task ones.  Tasks are launched from the last to the first, the m-th
+ * launch on streams[m % nstreams] (hipStream_t; NULL entries are the null
+ * stream).  When args->n_quartets, n_quartets_fp32 or quartet_hash is set it
+ * points to ntasks counters and task t adds into element t.  Stops at and
+ * returns the status of the first task that fails.  No counterpart in the
+ * reference, which calls RYS_build_jk once per task from Python. */
+int jk_build_tasks(const JKBuildArgs *args, const JKTask *tasks, int ntasks,
+                   void *const *streams, int nstreams);
+
 /* Primitive-pair records of a pair list, geometry-only data precomputed once
  * per molecule: out[(p*iprim*jprim + ip*jprim + jp)*6 + (0..5)] =
  * (a, 1/a, Px, Py, Pz, K) with a = ai + aj, P = (ai Ri + aj Rj)/a and

@@ -64,6 +64,24 @@ def test_jk_build_rejects_bad_arguments():
     assert b'repl_stride' in lib.jk_last_error()
 
 
+def test_jk_build_tasks_validates_on_host():
+    '''The batched entry checks its own arguments and stops at the first
+    task jk_build rejects, before any launch.'''
+    from gpu4pyscf_amd.lib import clib
+    lib = clib.load()
+    assert ctypes.sizeof(clib.JKTask) == 14 * 4 + 6 * 8
+    a = clib.JKBuildArgs()
+    a.abi_version = clib.JK_ABI_VERSION
+    tasks = (clib.JKTask * 2)()
+    streams = (ctypes.c_void_p * 1)(None)
+    assert lib.jk_build_tasks(ctypes.byref(a), tasks, 2, streams, 0) == 1
+    assert b'jk_build_tasks' in lib.jk_last_error()
+    tasks[1].li = 5                  # launched first (last task first)
+    assert lib.jk_build_tasks(ctypes.byref(a), tasks, 2, streams, 1) == 2
+    assert b'angular class' in lib.jk_last_error()
+    assert lib.jk_build_tasks(ctypes.byref(a), tasks, 0, streams, 1) == 0
+
+
 def test_tile_pairs_and_bins_match_oracle():
     from gpu4pyscf_amd.scf import jk
     from oracle import oracle as O
