@@ -83,6 +83,7 @@ NSTREAMS = 3
 # 1 -> 653, 2 -> 639, 4 -> 630, 8 -> 627, 16 -> 626; three streams:
 # 1 -> 562, 8 -> 549)
 NREPL = 8
+REPL_BYTES = 16 << 30
 
 
 def class_kernel_name(ls):
@@ -331,6 +332,8 @@ class _VHFOpt:
         # n_repl zeroed copies of vj/vk; workgroup b adds into copy
         # b % n_repl (one per XCD), the copies are summed after the build
         n_repl = max(int(self.n_repl), 1)
+        # at most REPL_BYTES for the copies of J and K together
+        n_repl = max(1, min(n_repl, REPL_BYTES // max(2 * dms.numel() * 8, 1)))
         shape = (n_repl,) + tuple(dms.shape)
         vj = torch.zeros(shape, dtype=dms.dtype, device=dms.device) if with_j else None
         vk = torch.zeros(shape, dtype=dms.dtype, device=dms.device) if with_k else None
