@@ -52,8 +52,10 @@
 #ifndef LANE_SKIP_NF
 #define LANE_SKIP_NF    0
 #endif
+// wavefronts per block of the per-lane kernel (32-water def2-TZVP build on
+// three streams, ms: 1 -> 546, 2 -> 464, 4 -> 489)
 #ifndef LANE_WAVES
-#define LANE_WAVES      4       // wavefronts per block of the per-lane kernel
+#define LANE_WAVES      2
 #endif
 // launch shapes: work items of the per-lane kernel target LANE_ITEMS blocks
 // per launch with at least LANE_MIN_CHUNK kl candidates each; the team
