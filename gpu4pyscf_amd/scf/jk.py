@@ -77,8 +77,10 @@ def group_by_split_points(q, split_points):
 
 LANE_MAX = 170      # LANE_MAX of csrc/jk_kernels.hip
 # class launches run on this many streams (32-water def2-TZVP build, ms:
-# 1 -> 648, 2 -> 569, 3 -> 564, 4 -> 566)
-NSTREAMS = 3
+# 1 -> 648, 2 -> 569, 3 -> 564, 4 -> 566 with four-wave per-lane blocks;
+# 2 -> 482, 3 -> 463, 4 -> 454 with two-wave blocks).  HIP gives a process
+# 4 hardware queues by default, so more streams would share them.
+NSTREAMS = 4
 # copies of vj/vk per build (32-water def2-TZVP build, ms, one stream:
 # 1 -> 653, 2 -> 639, 4 -> 630, 8 -> 627, 16 -> 626; three streams:
 # 1 -> 562, 8 -> 549)
