@@ -14,10 +14,10 @@ _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, 'libjkhip.so')
 TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
 
-JK_ABI_VERSION = 8
+JK_ABI_VERSION = 9
 JK_RYS_NMAX = 11
 JK_BOYS_M = 13
-JK_LMAX = 3
+JK_LMAX = 5
 
 c_void_p = ctypes.c_void_p
 
@@ -69,6 +69,8 @@ class JKBuildArgs(ctypes.Structure):
         ('quartet_hash', c_void_p),
         ('n_repl', ctypes.c_int),
         ('repl_stride', ctypes.c_int64),
+        ('omega', ctypes.c_double),
+        ('scale', ctypes.c_double),
     ]
 
 
@@ -81,6 +83,7 @@ class JKTask(ctypes.Structure):
 
 
 EXPORTS = ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
+           'jk_energy_ip1',
            'jk_int1e', 'mlp_gemm_bf16', 'mlp_silu_backward', 'mlp_colsum', 'mlp_mse_grad',
            'mlp_adamw', 'mlp_to_bf16', 'jk_last_error', 'jk_abi_version')
 
@@ -98,11 +101,14 @@ def load():
     lib = ctypes.CDLL(LIB_PATH)
     lib.jk_last_error.restype = ctypes.c_char_p
     for name in ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
-                 'jk_abi_version'):
+                 'jk_energy_ip1', 'jk_abi_version'):
         getattr(lib, name).restype = ctypes.c_int
     lib.jk_build.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p]
+    lib.jk_energy_ip1.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p, ctypes.c_double,
+                                  ctypes.c_double, c_void_p]
     lib.jk_pair_qcond.argtypes = [c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p,
-                                  ctypes.c_int, JKRysTable, c_void_p]
+                                  ctypes.c_int, JKRysTable, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, c_void_p]
     lib.jk_int1e.restype = ctypes.c_int
     lib.jk_int1e.argtypes = [c_void_p] * 6 + [ctypes.c_int, ctypes.c_int, c_void_p, c_void_p,
                                                ctypes.c_int, JKRysTable, c_void_p]

@@ -8,9 +8,12 @@ with the same attribute surface (``sorted_mol``, ``bas_pair_cache``,
 integral is evaluated by the HIP kernels of libjkhip.so.
 
 Accepted ``dm``: numpy arrays (returned as numpy) or torch tensors (returned
-as torch tensors on the same device).  omega != 0 raises NotImplementedError;
-shells with l > 3 raise NotImplementedError (the reference sends l > 4 to
-PySCF on the CPU, gpu4pyscf/scf/jk.py:622-640; no CPU path exists here).
+as torch tensors on the same device).  Range-separated operators (omega,
+lr_factor, sr_factor) follow gpu4pyscf/scf/jk.py:1122-1168.  Classes with
+shells up to l = 3 run on the templated per-lane/team kernels, l = 4 and 5
+on the runtime-l kernel; l > 5 raises NotImplementedError (the reference
+sends l > 4 to PySCF on the CPU, gpu4pyscf/scf/jk.py:622-640; no CPU path
+exists here).
 '''
 
 import ctypes
@@ -157,9 +160,12 @@ class _VHFOpt:
         mol = self.sorted_mol = SortedGTO(self.mol, diffuse_cutoff=0.3)
         lmax = int(mol.uniq_l_ctr[:, 0].max())
         if lmax > LMAX:
+            # the reference evaluates l > 4 shells with PySCF on the CPU
+            # (gpu4pyscf/scf/jk.py:622-640); here every class runs on the GPU
+            # up to l = 5 and higher l is refused
             raise NotImplementedError(f'l = {lmax} shells: the GPU path supports l <= {LMAX}')
-        if getattr(self.mol, 'omega', 0) not in (0, 0.):
-            raise NotImplementedError('range-separated Coulomb (omega != 0)')
+        # screening q_cond of the molecule's operator (full, erf or erfc)
+        q_omega, q_lr, q_sr = rsh_factors(self.mol)
         self.h_shls = []
         nbas = mol.nbas
         d = {}
@@ -200,7 +206,9 @@ class _VHFOpt:
             q_d = torch.empty(len(allp), dtype=torch.float32, device=dev)
             clib.check(lib.jk_pair_qcond(_ptr(q_d), _ptr(allp_d), ctypes.c_int(len(allp)),
                                          _ptr(d['bas']), _ptr(d['env']), ctypes.c_int(nbas),
-                                         rys, stream), 'jk_pair_qcond')
+                                         rys, ctypes.c_double(abs(q_omega)),
+                                         ctypes.c_double(q_lr), ctypes.c_double(q_sr), stream),
+                       'jk_pair_qcond')
             q_all = q_d.cpu().numpy()
         split = np.arange(math.log(self.direct_scf_tol), 2., Q_COND_MARGIN)
         cache = {}
@@ -294,10 +302,26 @@ class _VHFOpt:
             a.pdata_kl = d['pdata'][k, l].data_ptr()
         return arr
 
+    def operator_passes(self, omega=None, lr_factor=None, sr_factor=None):
+        """Launch passes [(omega >= 0, scale)] of the operator
+        lr_factor erf(|omega| r)/r + sr_factor erfc(|omega| r)/r
+        = sr_factor / r + (lr_factor - sr_factor) erf(|omega| r)/r;
+        omega=None takes the molecule's omega with the libcint sign
+        convention (gpu4pyscf/scf/jk.py:1122-1168)."""
+        omega, lr, sr = rsh_factors(self.mol, omega, lr_factor, sr_factor)
+        if omega == 0:
+            return [(0., float(lr))]
+        passes = []
+        if sr != 0:
+            passes.append((0., float(sr)))
+        if lr - sr != 0:
+            passes.append((abs(float(omega)), float(lr - sr)))
+        return passes
+
     def get_jk(self, dms, hermi, verbose=None, with_j=True, with_k=True,
                count_quartets=False, time_classes=False, fp32_tol=None,
-               hash_quartets=False):
-        '''J/K for density matrices ``dms`` (n_dm, nao_c, nao_c) in the sorted
+               hash_quartets=False, omega=None, lr_factor=None, sr_factor=None):
+        """J/K for density matrices ``dms`` (n_dm, nao_c, nao_c) in the sorted
         Cartesian basis (torch, float64, on the device).
 
         count_quartets: store the surviving quartets of each class task in
@@ -308,7 +332,11 @@ class _VHFOpt:
         time_classes: store (task, milliseconds) of each class launch, timed
         with HIP events on the stream it runs on, in ``self.last_task_ms``.
         fp32_tol: evaluate in FP32 the quartets whose bound
-        sqrt|(ij|ij)| sqrt|(kl|kl)| max|D| is below fp32_tol (None: all FP64).'''
+        sqrt|(ij|ij)| sqrt|(kl|kl)| max|D| is below fp32_tol (None: all FP64).
+        omega, lr_factor, sr_factor: the operator of J and K (default: the
+        molecule's omega); a range-separated operator takes two launches of
+        every class task (full range and erf-attenuated), whose counters are
+        those of the first."""
         torch = _torch()
         if self._dev is None:
             self.build()
@@ -362,8 +390,6 @@ class _VHFOpt:
         args.dm_penalty = dm_penalty
         args.n_dm = n_dm
         args.dm = dms.data_ptr()
-        args.vj = vj.data_ptr() if with_j else None
-        args.vk = vk.data_ptr() if with_k else None
         args.rys = self._rys
         args.fp32_cut = np.float32(math.log(fp32_tol)) if fp32_tol else np.float32(-1e30)
         args.n_repl = n_repl
@@ -372,6 +398,17 @@ class _VHFOpt:
         goff = mol.group_offsets
         g_ao0 = mol.ao_loc[goff[:-1]]
         g_nao = mol.ao_loc[goff[1:]] - g_ao0
+        # launch passes: (omega, scale, with J, with K); J and K share the
+        # launches when they use the same operator
+        passes = self.operator_passes(omega, lr_factor, sr_factor)
+        if omega is not None and with_j and with_k:
+            pj = self.operator_passes()
+            if pj != passes:
+                runs = [(w, f, True, False) for w, f in pj] + [(w, f, False, True) for w, f in passes]
+            else:
+                runs = [(w, f, True, True) for w, f in passes]
+        else:
+            runs = [(w, f, with_j, with_k) for w, f in passes]
         cur = torch.cuda.current_stream(dms.device)
         streams = [cur]
         if self.nstreams > 1:
@@ -384,59 +421,70 @@ class _VHFOpt:
             ready.record(cur)
             for s in streams[1:]:
                 s.wait_event(ready)
-        # launched from the last task (f classes, the longest) to the first,
-        # so that the short s/p classes fill the streams at the end
-        # (32-water def2-TZVP build: 521 ms in task order, 515 reversed).
-        # Without per-launch timing the whole loop is one native call.
-        if not (time_classes or prof):
-            if counter is not None:
-                args.n_quartets = counter.data_ptr()
-                args.n_quartets_fp32 = counter32.data_ptr()
-            if hashes is not None:
-                args.quartet_hash = hashes.data_ptr()
-            handles = (ctypes.c_void_p * len(streams))(*[s.cuda_stream for s in streams])
-            clib.check(lib.jk_build_tasks(ctypes.byref(args), d['task_array'], len(tasks),
-                                          handles, len(streams)), 'jk_build_tasks')
-            tasks_looped = []
-        else:
-            tasks_looped = range(len(tasks) - 1, -1, -1)
-        for n, t in enumerate(tasks_looped):
-            i, j, k, l = tasks[t]
-            task_stream = streams[n % len(streams)]
-            args.k_ao0, args.k_nao = int(g_ao0[k]), int(g_nao[k])
-            args.l_ao0, args.l_nao = int(g_ao0[l]), int(g_nao[l])
-            if counter is not None:
-                args.n_quartets = counter.data_ptr() + 8 * t
-                args.n_quartets_fp32 = counter32.data_ptr() + 8 * t
-            if hashes is not None:
-                args.quartet_hash = hashes.data_ptr() + 8 * t
-            pij, qij = d['pairs'][i, j], d['q'][i, j]
-            pkl, qkl = d['pairs'][k, l], d['q'][k, l]
-            args.li, args.lj, args.lk, args.ll = (int(uniq[x, 0]) for x in (i, j, k, l))
-            args.iprim, args.jprim, args.kprim, args.lprim = (int(uniq[x, 1]) for x in (i, j, k, l))
-            args.npairs_ij = len(pij)
-            args.npairs_kl = len(pkl)
-            args.pairs_ij = pij.data_ptr()
-            args.pairs_kl = pkl.data_ptr()
-            args.q_ij = qij.data_ptr()
-            args.q_kl = qkl.data_ptr()
-            args.pdata_ij = d['pdata'][i, j].data_ptr()
-            args.pdata_kl = d['pdata'][k, l].data_ptr()
-            if time_classes or prof:
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record(task_stream)
-            st = lib.jk_build(ctypes.byref(args), ctypes.c_void_p(task_stream.cuda_stream))
-            if time_classes or prof:
-                e1.record(task_stream)
-                events.append((t, e0, e1))
-            if st != 0:
-                clib.check(st, f'jk_build ({"spdfg"[args.li]}{"spdfg"[args.lj]}|'
-                               f'{"spdfg"[args.lk]}{"spdfg"[args.ll]})')
-        for s in streams[1:]:
-            done = torch.cuda.Event()
-            done.record(s)
-            cur.wait_event(done)
+        # the side streams are joined back into the caller's stream even when a
+        # launch fails, so that no kernel still adds into vj/vk after they are
+        # released
+        try:
+            for n_run, (w, f, rj, rk) in enumerate(runs):
+                args.omega = w
+                args.scale = f
+                args.vj = vj.data_ptr() if rj else None
+                args.vk = vk.data_ptr() if rk else None
+                first = n_run == 0
+                # launched from the last task (f classes, the longest) to the
+                # first, so that the short s/p classes fill the streams at the
+                # end (32-water def2-TZVP build: 521 ms in task order, 515
+                # reversed).  Without per-launch timing the whole loop is one
+                # native call.
+                if not (time_classes or prof):
+                    args.n_quartets = counter.data_ptr() if (first and counter is not None) else None
+                    args.n_quartets_fp32 = (counter32.data_ptr()
+                                            if (first and counter is not None) else None)
+                    args.quartet_hash = hashes.data_ptr() if (first and hashes is not None) else None
+                    handles = (ctypes.c_void_p * len(streams))(*[s.cuda_stream for s in streams])
+                    clib.check(lib.jk_build_tasks(ctypes.byref(args), d['task_array'], len(tasks),
+                                                  handles, len(streams)), 'jk_build_tasks')
+                    continue
+                for n, t in enumerate(range(len(tasks) - 1, -1, -1)):
+                    i, j, k, l = tasks[t]
+                    task_stream = streams[n % len(streams)]
+                    args.k_ao0, args.k_nao = int(g_ao0[k]), int(g_nao[k])
+                    args.l_ao0, args.l_nao = int(g_ao0[l]), int(g_nao[l])
+                    args.n_quartets = (counter.data_ptr() + 8 * t
+                                       if (first and counter is not None) else None)
+                    args.n_quartets_fp32 = (counter32.data_ptr() + 8 * t
+                                            if (first and counter is not None) else None)
+                    args.quartet_hash = (hashes.data_ptr() + 8 * t
+                                         if (first and hashes is not None) else None)
+                    pij, qij = d['pairs'][i, j], d['q'][i, j]
+                    pkl, qkl = d['pairs'][k, l], d['q'][k, l]
+                    args.li, args.lj, args.lk, args.ll = (int(uniq[x, 0]) for x in (i, j, k, l))
+                    args.iprim, args.jprim, args.kprim, args.lprim = (int(uniq[x, 1])
+                                                                      for x in (i, j, k, l))
+                    args.npairs_ij = len(pij)
+                    args.npairs_kl = len(pkl)
+                    args.pairs_ij = pij.data_ptr()
+                    args.pairs_kl = pkl.data_ptr()
+                    args.q_ij = qij.data_ptr()
+                    args.q_kl = qkl.data_ptr()
+                    args.pdata_ij = d['pdata'][i, j].data_ptr()
+                    args.pdata_kl = d['pdata'][k, l].data_ptr()
+                    if first and (time_classes or prof):
+                        e0 = torch.cuda.Event(enable_timing=True)
+                        e1 = torch.cuda.Event(enable_timing=True)
+                        e0.record(task_stream)
+                    st = lib.jk_build(ctypes.byref(args), ctypes.c_void_p(task_stream.cuda_stream))
+                    if first and (time_classes or prof):
+                        e1.record(task_stream)
+                        events.append((t, e0, e1))
+                    if st != 0:
+                        clib.check(st, f'jk_build ({"spdfghi"[args.li]}{"spdfghi"[args.lj]}|'
+                                       f'{"spdfghi"[args.lk]}{"spdfghi"[args.ll]})')
+        finally:
+            for s in streams[1:]:
+                done = torch.cuda.Event()
+                done.record(s)
+                cur.wait_event(done)
         if with_j:
             vj = vj[0] if n_repl == 1 else vj.sum(0)
         if with_k:
@@ -475,6 +523,39 @@ class _VHFOpt:
         return vj, vk
 
 
+def rsh_factors(mol, omega=None, lr_factor=None, sr_factor=None):
+    """(omega, lr_factor, sr_factor) of the operator
+    lr_factor erf(|omega| r)/r + sr_factor erfc(|omega| r)/r, following
+    _check_rsh_factors (gpu4pyscf/scf/jk.py:1122-1168): omega None takes
+    mol.omega; without factors omega > 0 is long range, omega < 0 short
+    range."""
+    if omega is None:
+        omega = getattr(mol, 'omega', 0.) or 0.
+    if lr_factor is None and sr_factor is None:
+        if omega == 0:
+            lr_factor = sr_factor = 1
+        elif omega < 0:
+            lr_factor, sr_factor = 0, 1
+        else:
+            lr_factor, sr_factor = 1, 0
+    elif lr_factor is None:
+        lr_factor = sr_factor if omega == 0 else 0
+    elif sr_factor is None:
+        if omega == 0:
+            sr_factor = lr_factor
+        else:
+            assert omega > 0
+            sr_factor = 0
+    else:
+        if omega == 0:
+            assert lr_factor == sr_factor
+        elif lr_factor == sr_factor:
+            omega = 0
+    if sr_factor != 0 and omega != 0:
+        omega = -abs(omega)
+    return omega, lr_factor, sr_factor
+
+
 def _to_device(dm):
     torch = _torch()
     if isinstance(dm, torch.Tensor):
@@ -484,15 +565,13 @@ def _to_device(dm):
 
 def get_jk(mol, dm, hermi=0, vhfopt=None, with_j=True, with_k=True,
            omega=None, lr_factor=None, sr_factor=None, verbose=None):
-    '''Coulomb and exchange matrices, gpu4pyscf/scf/jk.py:60-151.
+    """Coulomb and exchange matrices, gpu4pyscf/scf/jk.py:60-151.
 
     Returns (vj, vk) with the shape of ``dm``; an element is None when not
-    requested.'''
+    requested.  With omega (and lr_factor, sr_factor) given, K uses the
+    operator lr_factor erf(|omega| r)/r + sr_factor erfc(|omega| r)/r and J
+    the molecule's operator (mol.omega), as in the reference."""
     assert with_j or with_k
-    if omega is None:
-        omega = getattr(mol, 'omega', 0.)
-    if omega != 0 or lr_factor is not None or sr_factor is not None:
-        raise NotImplementedError('range-separated Coulomb (omega != 0)')
     if vhfopt is None:
         vhfopt = _VHFOpt(mol).build()
     dm_t, was_torch = _to_device(dm)
@@ -500,7 +579,8 @@ def get_jk(mol, dm, hermi=0, vhfopt=None, with_j=True, with_k=True,
     nao0 = shape[-1]
     dms = dm_t.reshape(-1, nao0, nao0)
     dms = vhfopt.apply_coeff_C_mat_CT(dms)
-    vj, vk = vhfopt.get_jk(dms, hermi, verbose, with_j=with_j, with_k=with_k)
+    vj, vk = vhfopt.get_jk(dms, hermi, verbose, with_j=with_j, with_k=with_k,
+                           omega=omega, lr_factor=lr_factor, sr_factor=sr_factor)
     if with_j:
         vj = vhfopt.apply_coeff_CT_mat_C(vj).reshape(shape)
     if with_k:

@@ -24,9 +24,9 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  8
+#define JK_ABI_VERSION  9
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
-#define JK_LMAX         3       /* max angular momentum on the GPU path */
+#define JK_LMAX         5       /* max angular momentum on the GPU path */
 
 /* Chebyshev tables of Rys roots/weights (device pointer + layout).
  * coef[off[n-1] + (k*2n + f)*(degree+1) + d] on unit intervals [k, k+1),
@@ -101,11 +101,33 @@ typedef struct {
      * reference. */
     int n_repl;
     int64_t repl_stride;
+    /* Operator: scale * erf(omega r)/r for omega > 0, scale / r for
+     * omega == 0 (scale multiplies every integral of the launch).  The
+     * range-separated operators of gpu4pyscf/scf/jk.py:1122-1168,
+     * lr erf(w r)/r + sr erfc(w r)/r = sr / r + (lr - sr) erf(w r)/r, are
+     * built from two launches of each class task; the reference instead
+     * doubles the Rys roots (rys_roots_for_k.cu:19-68). */
+    double omega;
+    double scale;           /* 0 is taken as 1 */
 } JKBuildArgs;
 
 /* Accumulate one class task into vj/vk (before the hermi post-processing of
  * gpu4pyscf/scf/jk.py:594-616).  stream: hipStream_t or NULL. */
 int jk_build(const JKBuildArgs *args, void *stream);
+
+/* First derivatives of the two-electron energy per atom for one class task:
+ * ejk[atom*3 + x] += d/dR_atom,x of
+ *   sum over the task's surviving quartets of (ij|kl) Gamma_ijkl,
+ * Gamma = j_factor J-part - k_factor K-part of the two-particle density of
+ * the n_dm (1: closed shell, 2: alpha and beta) densities in args->dm, with
+ * the factors and the screening of RYS_per_atom_jk_ip1
+ * (gpu4pyscf/lib/gvhf-rys/rys_contract_jk_ip1.cu:1747-1800,
+ * create_tasks.cu:829-889); args->dm_cond is the block log max of the
+ * densities and args->dm_penalty is ignored.  ejk: natm x 3 FP64 on the
+ * device, zeroed by the caller; atoms are read from bas[ATOM_OF].
+ * Replaces RYS_per_atom_jk_ip1, bound by gpu4pyscf/grad/rhf.py:106-154. */
+int jk_energy_ip1(const JKBuildArgs *args, double *ejk, double j_factor, double k_factor,
+                  void *stream);
 
 /* The per-task fields of JKBuildArgs, for a whole build in one call. */
 typedef struct {
@@ -150,12 +172,15 @@ int jk_ovlp_mask(int8_t *mask, const float *exps, const float *log_coeff,
                  const double *coords, const int32_t *ls, int nbas,
                  float log_cutoff, void *stream);
 
-/* q[p] = float32(log(max |(ij|ij)|) / 2), -700 if zero, for pairs p.
+/* q[p] = float32(log(max |(ij|ij)|) / 2), -700 if zero, for pairs p, of
+ * the operator sr_factor / r + (lr_factor - sr_factor) erf(omega r)/r
+ * (omega = 0: lr_factor / r); shells up to l = 5; evaluated in FP64.
  * Replaces int2e_qcond_estimator (gpu4pyscf/lib/gvhf-rys/
- * nr_sr_estimator.cu:416-430) for omega = 0; evaluated in FP64. */
+ * nr_sr_estimator.cu:416-430). */
 int jk_pair_qcond(float *q, const uint32_t *pairs, int npairs,
                   const int32_t *bas, const double *env, int nbas,
-                  JKRysTable rys, void *stream);
+                  JKRysTable rys, double omega, double lr_factor, double sr_factor,
+                  void *stream);
 
 /* dm_cond[ish,jsh] = float32(log(max_{dm, u in ish, v in jsh} |D| + 1e-300)),
  * with the block maxima symmetrised (A + A^T) first when hermi == 0.

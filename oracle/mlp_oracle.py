@@ -1,5 +1,7 @@
 '''CPU oracle of the learned-guess network kernels -- TEST INFRASTRUCTURE ONLY.
 
+Only tests/ use it; the product never imports it.
+
 Forward, backward and AdamW of the pair MLP in numpy (float64 arithmetic),
 rounding to bfloat16 at exactly the points where the HIP kernels store
 bfloat16 (inputs, weights, hidden activations, dL/dZ), so the kernels' FP32
@@ -27,36 +29,48 @@ def silu(z):
     return z / (1. + np.exp(-z))
 
 
-def forward(x, Ws, bs):
-    a = bf16(x)
+def exact(x):
+    '''No rounding: with rnd=exact the oracle is the float64 network, which
+    the tests compare with torch autograd.'''
+    return np.asarray(x, dtype=np.float64)
+
+
+def forward(x, Ws, bs, rnd=bf16):
+    a = rnd(x)
     zs, acts = [], [a]
     for l, (W, b) in enumerate(zip(Ws, bs)):
-        z = acts[-1] @ bf16(W) + b
+        z = acts[-1] @ rnd(W) + b
         zs.append(z)
         if l < len(Ws) - 1:
-            acts.append(bf16(silu(z)))
+            acts.append(rnd(silu(z)))
     return zs[-1], zs, acts
 
 
-def backward(dy, zs, acts, Ws):
+def backward(dy, zs, acts, Ws, rnd=bf16):
     '''dy: dL/dY (float); returns (dWs, dbs).'''
     dWs = [None] * len(Ws)
     dbs = [None] * len(Ws)
     d = dy
-    d16 = bf16(dy)
+    d16 = rnd(dy)
     for l in range(len(Ws) - 1, -1, -1):
         if l == len(Ws) - 1:
             dbs[l] = d.sum(axis=0)
         dWs[l] = acts[l].T @ d16
         if l == 0:
             break
-        dA = d16 @ bf16(Ws[l]).T
+        dA = d16 @ rnd(Ws[l]).T
         z = zs[l - 1]
         s = 1. / (1. + np.exp(-z))
         dz = dA * s * (1. + z * (1. - s))
         dbs[l - 1] = dz.sum(axis=0)
-        d16 = bf16(dz)
+        d16 = rnd(dz)
     return dWs, dbs
+
+
+def mse_grad(y, t, mask, scale):
+    '''Masked MSE: loss = scale sum mask (y - t)^2 and dloss/dy.'''
+    r = mask * (y - t)
+    return scale * float(np.sum(r * (y - t))), 2. * scale * r
 
 
 def adamw(w, m, v, g, step, lr, b1=.9, b2=.999, eps=1e-8, wd=1e-4):

@@ -43,7 +43,33 @@ def lib():
         _lib.oracle_jk.restype = ctypes.c_long
         _lib.oracle_splitmix64_mix.restype = ctypes.c_uint64
         _lib.oracle_splitmix64_mix.argtypes = [ctypes.c_uint64]
+        _lib.oracle_set_operator.argtypes = [ctypes.c_double] * 3
+        _lib.oracle_ejk_ip1.argtypes = ([ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 2 +
+                                        [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int] +
+                                        [ctypes.c_double] * 2 + [ctypes.c_void_p])
     return _lib
+
+
+def rsh_factors(omega, lr_factor=None, sr_factor=None):
+    '''(omega, lr, sr) of lr erf(|omega| r)/r + sr erfc(|omega| r)/r:
+    libcint's sign convention when no factor is given (omega > 0 long range,
+    omega < 0 short range), gpu4pyscf/scf/jk.py:1122-1168.'''
+    omega = omega or 0.
+    if lr_factor is None and sr_factor is None:
+        if omega == 0:
+            return 0., 1., 1.
+        return (abs(omega), 0., 1.) if omega < 0 else (omega, 1., 0.)
+    if lr_factor is None:
+        lr_factor = sr_factor if omega == 0 else 0.
+    if sr_factor is None:
+        sr_factor = lr_factor if omega == 0 else 0.
+    return abs(omega), lr_factor, sr_factor
+
+
+def set_operator(omega=0., lr_factor=None, sr_factor=None):
+    '''Operator of every later ERI of the oracle (a process-wide setting).'''
+    w, lr, sr = rsh_factors(omega, lr_factor, sr_factor)
+    lib().oracle_set_operator(w, lr, sr)
 
 
 def _p(a):
@@ -296,24 +322,64 @@ def splitmix64_mix(z):
 
 
 def get_jk(mol, dm, hermi=1, with_j=True, with_k=True, direct_scf_tol=1e-13,
-           sorted_mol=None, cache=None):
-    '''Oracle J/K in the original AO basis of ``mol``.'''
+           sorted_mol=None, cache=None, omega=None, lr_factor=None, sr_factor=None):
+    """Oracle J/K in the original AO basis of ``mol``.  The screening uses
+    the molecule's operator (mol.omega); K uses (omega, lr_factor,
+    sr_factor) when omega is given, J always the molecule's operator."""
     from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    mol_omega = getattr(mol, 'omega', 0.) or 0.
     if sorted_mol is None:
         sorted_mol = SortedGTO(mol)
-    if cache is None:
-        cache = build_pairs(sorted_mol, direct_scf_tol)
-    dm = np.asarray(dm, dtype=np.float64)
-    shape = dm.shape
-    nao0 = shape[-1]
-    dms = dm.reshape(-1, nao0, nao0)
-    C = sorted_mol.coeff
-    dms_c = np.einsum('pi,nij,qj->npq', C, dms, C)
-    vj, vk, nsurv, _ = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol,
-                                 with_j, with_k)
+    try:
+        set_operator(mol_omega)
+        if cache is None:
+            cache = build_pairs(sorted_mol, direct_scf_tol)
+        dm = np.asarray(dm, dtype=np.float64)
+        shape = dm.shape
+        nao0 = shape[-1]
+        dms = dm.reshape(-1, nao0, nao0)
+        C = sorted_mol.coeff
+        dms_c = np.einsum('pi,nij,qj->npq', C, dms, C)
+        if omega is None or rsh_factors(omega, lr_factor, sr_factor) == rsh_factors(mol_omega):
+            vj, vk, nsurv, _ = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol,
+                                         with_j, with_k)
+        else:
+            vj = vk = None
+            if with_j:
+                vj = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol, True, False)[0]
+            if with_k:
+                set_operator(omega, lr_factor, sr_factor)
+                vk = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol, False, True)[1]
+    finally:
+        set_operator(0.)
     vj = np.einsum('pi,npq,qj->nij', C, vj, C).reshape(shape) if with_j else None
     vk = np.einsum('pi,npq,qj->nij', C, vk, C).reshape(shape) if with_k else None
     return vj, vk
+
+
+def ejk_ip1(mol, dm, j_factor=1., k_factor=1.):
+    """Per-atom first derivatives of the two-electron energy (natm x 3) for
+    one (closed shell) or two (alpha, beta) densities in the original AO
+    basis, with the molecule's operator; no screening."""
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    sm = SortedGTO(mol)
+    dm = np.asarray(dm, dtype=np.float64)
+    dms = dm.reshape(-1, dm.shape[-1], dm.shape[-1])
+    C = sm.coeff
+    dms_c = np.ascontiguousarray(np.einsum('pi,nij,qj->npq', C, dms, C))
+    bas = np.ascontiguousarray(sm._bas, dtype=np.int32)
+    env = np.ascontiguousarray(sm.env_kernel)
+    aol = np.ascontiguousarray(sm.ao_loc, dtype=np.int32)
+    atom_of = np.ascontiguousarray(sm._bas[:, 0], dtype=np.int32)
+    out = np.zeros((mol.natm, 3))
+    try:
+        set_operator(getattr(mol, 'omega', 0.) or 0.)
+        lib().oracle_ejk_ip1(dms_c.shape[0], sm.nao, _p(dms_c), _p(bas), sm.nbas, _p(env),
+                             _p(aol), _p(atom_of), mol.natm, float(j_factor), float(k_factor),
+                             _p(out))
+    finally:
+        set_operator(0.)
+    return out
 
 
 def fp(a):

@@ -185,25 +185,48 @@ static void cart_pow(int l, int *px, int *py, int *pz)
         }
 }
 
-/* Cartesian (ij|kl) block of four sorted shells (one contraction each),
- * layout out[((i*nfj + j)*nfk + k)*nfl + l]. */
-void oracle_eri_shell(const int *bas, const double *env,
-                      int ish, int jsh, int ksh, int lsh, double *out)
+/* Operator of the ERIs: w_full / r + w_lr erf(omega r)/r.  The long-range
+ * part uses the Rys rule of the attenuated operator: with
+ * a = omega^2 / (omega^2 + theta) the roots and weights of x a, scaled by a
+ * and sqrt(a) (the substitution t = t_max tau, t_max^2 = a, of the Rys
+ * integral over t in [0, t_max]).  Set once before the calls, not thread
+ * local. */
+static double g_omega = 0., g_w_full = 1., g_w_lr = 0.;
+
+/* lr erf(omega r)/r + sr erfc(omega r)/r = sr / r + (lr - sr) erf(omega r)/r
+ * (omega > 0); omega == 0: lr / r. */
+void oracle_set_operator(double omega, double lr, double sr)
 {
-    const int *bi = bas + ish*BAS_SLOTS, *bj = bas + jsh*BAS_SLOTS;
-    const int *bk = bas + ksh*BAS_SLOTS, *bl = bas + lsh*BAS_SLOTS;
-    int li = bi[ANG_OF], lj = bj[ANG_OF], lk = bk[ANG_OF], ll = bl[ANG_OF];
+    g_omega = omega > 0 ? omega : 0.;
+    g_w_full = omega > 0 ? sr : lr;
+    g_w_lr = omega > 0 ? lr - sr : 0.;
+}
+
+/* one contracted Cartesian shell */
+typedef struct { int l, nprim; const double *exps, *coefs, *R; } Shell;
+
+static Shell shell_of(const int *bas, const double *env, int sh)
+{
+    const int *b = bas + sh*BAS_SLOTS;
+    Shell s = {b[ANG_OF], b[NPRIM_OF], env + b[PTR_EXP], env + b[PTR_COEFF], env + b[PTR_BAS_COORD]};
+    return s;
+}
+
+/* Cartesian (ij|kl) block of four shells, layout
+ * out[((i*nfj + j)*nfk + k)*nfl + l]. */
+static void eri_gen(const Shell *si, const Shell *sj, const Shell *sk, const Shell *sl,
+                    double *out)
+{
+    int li = si->l, lj = sj->l, lk = sk->l, ll = sl->l;
     int nfi = ncart(li), nfj = ncart(lj), nfk = ncart(lk), nfl = ncart(ll);
     int lij = li + lj, lkl = lk + ll;
     int nroots = (lij + lkl) / 2 + 1;
-    const double *ri = env + bi[PTR_BAS_COORD], *rj = env + bj[PTR_BAS_COORD];
-    const double *rk = env + bk[PTR_BAS_COORD], *rl = env + bl[PTR_BAS_COORD];
+    const double *ri = si->R, *rj = sj->R, *rk = sk->R, *rl = sl->R;
     double rij[3], rkl[3];
     for (int d = 0; d < 3; d++) { rij[d] = rj[d] - ri[d]; rkl[d] = rl[d] - rk[d]; }
     double rr_ij = rij[0]*rij[0] + rij[1]*rij[1] + rij[2]*rij[2];
     double rr_kl = rkl[0]*rkl[0] + rkl[1]*rkl[1] + rkl[2]*rkl[2];
-    int pi_[28], pj_[3][28], ip[3][28], jp[3][28], kp[3][28], lp[3][28];
-    (void)pi_; (void)pj_;
+    int ip[3][36], jp[3][36], kp[3][36], lp[3][36];
     cart_pow(li, ip[0], ip[1], ip[2]);
     cart_pow(lj, jp[0], jp[1], jp[2]);
     cart_pow(lk, kp[0], kp[1], kp[2]);
@@ -212,32 +235,42 @@ void oracle_eri_shell(const int *bas, const double *env,
     memset(out, 0, sizeof(double) * nout);
 
     /* G[d][a][c] for a<=lij, c<=lkl; g[d][i][j][k][l] after HRR */
-    enum { A1 = 2*MAXL+1 };
+    enum { A1 = 2*MAXL+3 };
     static __thread double G[3][A1][A1];
-    static __thread double H1[3][A1][MAXL+1][A1];        /* [i][j][c] */
-    static __thread double g[3][MAXL+1][MAXL+1][MAXL+1][MAXL+1];
+    static __thread double H1[3][A1][MAXL+2][A1];        /* [i][j][c] */
+    static __thread double g[3][MAXL+2][MAXL+2][MAXL+2][MAXL+2];
     double u[MAXROOTS], w[MAXROOTS];
 
-    for (int pi = 0; pi < bi[NPRIM_OF]; pi++)
-    for (int pj = 0; pj < bj[NPRIM_OF]; pj++) {
-        double ai = env[bi[PTR_EXP]+pi], aj = env[bj[PTR_EXP]+pj];
+    for (int pi = 0; pi < si->nprim; pi++)
+    for (int pj = 0; pj < sj->nprim; pj++) {
+        double ai = si->exps[pi], aj = sj->exps[pj];
         double aij = ai + aj;
         double Kab = exp(-ai * aj / aij * rr_ij);
-        double cicj = env[bi[PTR_COEFF]+pi] * env[bj[PTR_COEFF]+pj] * Kab;
+        double cicj = si->coefs[pi] * sj->coefs[pj] * Kab;
         double P[3];
         for (int d = 0; d < 3; d++) P[d] = ri[d] + rij[d] * aj / aij;
-        for (int pk = 0; pk < bk[NPRIM_OF]; pk++)
-        for (int pl = 0; pl < bl[NPRIM_OF]; pl++) {
-            double ak = env[bk[PTR_EXP]+pk], al = env[bl[PTR_EXP]+pl];
+        for (int pk = 0; pk < sk->nprim; pk++)
+        for (int pl = 0; pl < sl->nprim; pl++) {
+            double ak = sk->exps[pk], al = sl->exps[pl];
             double akl = ak + al;
             double Kcd = exp(-ak * al / akl * rr_kl);
-            double ckcl = env[bk[PTR_COEFF]+pk] * env[bl[PTR_COEFF]+pl] * Kcd;
+            double ckcl = sk->coefs[pk] * sl->coefs[pl] * Kcd;
             double Q[3], PQ[3];
             for (int d = 0; d < 3; d++) { Q[d] = rk[d] + rkl[d] * al / akl; PQ[d] = P[d] - Q[d]; }
             double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
             double theta = aij * akl / (aij + akl);
-            double fac = PI_FAC * cicj * ckcl / (aij * akl * sqrt(aij + akl));
-            oracle_rys_roots(nroots, theta * rr, u, w);
+            double fac0 = PI_FAC * cicj * ckcl / (aij * akl * sqrt(aij + akl));
+            for (int part = 0; part < 2; part++) {
+            double wpart = part == 0 ? g_w_full : g_w_lr;
+            if (wpart == 0.) continue;
+            double fac = fac0 * wpart;
+            if (part == 0) {
+                oracle_rys_roots(nroots, theta * rr, u, w);
+            } else {
+                double aw = g_omega * g_omega / (g_omega * g_omega + theta);
+                oracle_rys_roots(nroots, theta * rr * aw, u, w);
+                for (int ir = 0; ir < nroots; ir++) { u[ir] *= aw; w[ir] *= sqrt(aw); }
+            }
             for (int ir = 0; ir < nroots; ir++) {
                 double rt = u[ir];
                 double rt_aa = rt / (aij + akl);
@@ -272,7 +305,7 @@ void oracle_eri_shell(const int *bas, const double *env,
                     /* HRR on the ket */
                     for (int a = 0; a <= li; a++)
                     for (int b = 0; b <= lj; b++) {
-                        double tmp[A1][MAXL+1];
+                        double tmp[A1][MAXL+2];
                         for (int c = 0; c <= lkl; c++) tmp[c][0] = H1[d][a][b][c];
                         for (int e = 0; e < ll; e++)
                             for (int c = 0; c <= lkl - e - 1; c++)
@@ -292,7 +325,144 @@ void oracle_eri_shell(const int *bas, const double *env,
                             * g[2][ip[2][i]][jp[2][j]][kp[2][k]][lp[2][l]];
                 }
             }
+            }
         }
+    }
+}
+
+/* Cartesian (ij|kl) block of four sorted shells (one contraction each),
+ * layout out[((i*nfj + j)*nfk + k)*nfl + l]. */
+void oracle_eri_shell(const int *bas, const double *env,
+                      int ish, int jsh, int ksh, int lsh, double *out)
+{
+    Shell a = shell_of(bas, env, ish), b = shell_of(bas, env, jsh);
+    Shell c = shell_of(bas, env, ksh), d = shell_of(bas, env, lsh);
+    eri_gen(&a, &b, &c, &d, out);
+}
+
+/* index of the Cartesian component (x, y, z) of a shell of l = x + y + z */
+static inline int cart_index(int x, int y, int z)
+{
+    int l = x + y + z;
+    return (l - x) * (l - x + 1) / 2 + (l - x - y);
+}
+
+/*
+ * First derivatives of the two-electron energy per atom,
+ *   ejk[A] = d/dR_A sum over unique quartets (ij|kl) Gamma_ijkl
+ * with the two-particle density and factors of RYS_per_atom_jk_ip1
+ * (gpu4pyscf/lib/gvhf-rys/rys_contract_jk_ip1.cu:523-577, 1789-1797):
+ * Gamma = fac_sym (4 j_factor D_ji D_lk - 2 k_factor' (D_jk D_li + D_jl D_ki)),
+ * summed over the n_dm (1 or 2) densities as there (k_factor' = k_factor/2
+ * for one density), fac_sym = 1/2 for ish == jsh, ksh == lsh, ij == kl.
+ * Derivatives: d/dA_x phi_a = 2 alpha phi_{a+1x} - a_x phi_{a-1x} at the
+ * primitive level, evaluated as the ERI blocks of a shell of l+1 with
+ * coefficients 2 alpha c and of l-1 with c; centre l by translational
+ * invariance.  Every quartet of the sorted basis, no screening.  atom_of:
+ * atom of each shell; ejk: natm x 3, accumulated.
+ */
+void oracle_ejk_ip1(int n_dm, int nao, const double *dms, const int *bas, int nbas,
+                    const double *env, const int *ao_loc, const int *atom_of, int natm,
+                    double j_factor, double k_factor, double *ejk)
+{
+    size_t nao2 = (size_t)nao * nao;
+    if (n_dm == 1) k_factor *= .5;
+    const double jf = 4. * j_factor, kf = -2. * k_factor;
+    int npair = nbas * (nbas + 1) / 2;
+#pragma omp parallel
+    {
+        double *acc = calloc((size_t)natm * 3, sizeof(double));
+        double *bp = malloc(sizeof(double) * 36*36*36*36);
+        double *bm = malloc(sizeof(double) * 36*36*36*36);
+        double *c2a = malloc(sizeof(double) * 64);
+#pragma omp for schedule(dynamic, 1)
+        for (int pij = 0; pij < npair; pij++) {
+            int ish = (int)((sqrt(8. * pij + 1.) - 1.) / 2.);
+            while (ish * (ish + 1) / 2 > pij) ish--;
+            while ((ish + 1) * (ish + 2) / 2 <= pij) ish++;
+            int jsh = pij - ish * (ish + 1) / 2;
+            for (int ksh = 0; ksh <= ish; ksh++)
+            for (int lsh = 0; lsh <= ksh; lsh++) {
+                int pkl = ksh * (ksh + 1) / 2 + lsh;
+                if (pkl > pij) continue;
+                Shell sh[4] = {shell_of(bas, env, ish), shell_of(bas, env, jsh),
+                               shell_of(bas, env, ksh), shell_of(bas, env, lsh)};
+                double fac = 1.;
+                if (ish == jsh) fac *= .5;
+                if (ksh == lsh) fac *= .5;
+                if (pij == pkl) fac *= .5;
+                int off[4] = {ao_loc[ish], ao_loc[jsh], ao_loc[ksh], ao_loc[lsh]};
+                int nf[4], pw[4][3][36];
+                for (int c = 0; c < 4; c++) {
+                    nf[c] = ncart(sh[c].l);
+                    cart_pow(sh[c].l, pw[c][0], pw[c][1], pw[c][2]);
+                }
+                int atoms[4] = {atom_of[ish], atom_of[jsh], atom_of[ksh], atom_of[lsh]};
+                double grad[3][3] = {{0}};          /* centre (i, j, k) x direction */
+                for (int cen = 0; cen < 3; cen++) {
+                    /* blocks with the centre's shell raised and lowered */
+                    Shell up = sh[cen], dn = sh[cen];
+                    up.l += 1;
+                    for (int p = 0; p < up.nprim; p++) c2a[p] = 2. * up.exps[p] * up.coefs[p];
+                    up.coefs = c2a;
+                    Shell q[4] = {sh[0], sh[1], sh[2], sh[3]};
+                    q[cen] = up;
+                    eri_gen(&q[0], &q[1], &q[2], &q[3], bp);
+                    int has_dn = sh[cen].l > 0;
+                    if (has_dn) {
+                        dn.l -= 1;
+                        q[cen] = dn;
+                        eri_gen(&q[0], &q[1], &q[2], &q[3], bm);
+                    }
+                    int nfu[4] = {nf[0], nf[1], nf[2], nf[3]}, nfd[4] = {nf[0], nf[1], nf[2], nf[3]};
+                    nfu[cen] = ncart(sh[cen].l + 1);
+                    nfd[cen] = has_dn ? ncart(sh[cen].l - 1) : 1;
+                    for (int i = 0; i < nf[0]; i++)
+                    for (int j = 0; j < nf[1]; j++)
+                    for (int k = 0; k < nf[2]; k++)
+                    for (int l = 0; l < nf[3]; l++) {
+                        size_t I = off[0] + i, J = off[1] + j, K = off[2] + k, L = off[3] + l;
+                        double gam = 0.;
+                        for (int id = 0; id < n_dm; id++) {
+                            const double *D = dms + id * nao2;
+                            gam += kf * (D[J*nao+K] * D[L*nao+I] + D[J*nao+L] * D[K*nao+I]);
+                        }
+                        double dji = 0., dlk = 0.;
+                        for (int id = 0; id < n_dm; id++) {
+                            dji += dms[id * nao2 + J*nao+I]; dlk += dms[id * nao2 + L*nao+K];
+                        }
+                        gam += jf * dji * dlk;
+                        gam *= fac;
+                        int idx[4] = {i, j, k, l};
+                        for (int x = 0; x < 3; x++) {
+                            int p0 = pw[cen][0][idx[cen]], p1 = pw[cen][1][idx[cen]],
+                                p2 = pw[cen][2][idx[cen]];
+                            int pp[3] = {p0, p1, p2};
+                            pp[x] += 1;
+                            int iu[4] = {i, j, k, l}, id_[4] = {i, j, k, l};
+                            iu[cen] = cart_index(pp[0], pp[1], pp[2]);
+                            double v = bp[((iu[0]*nfu[1] + iu[1])*nfu[2] + iu[2])*nfu[3] + iu[3]];
+                            pp[x] -= 2;
+                            if (pp[x] >= 0) {
+                                id_[cen] = cart_index(pp[0], pp[1], pp[2]);
+                                v -= (pp[x] + 1) *
+                                     bm[((id_[0]*nfd[1] + id_[1])*nfd[2] + id_[2])*nfd[3] + id_[3]];
+                            }
+                            grad[cen][x] += gam * v;
+                        }
+                    }
+                }
+                for (int x = 0; x < 3; x++) {
+                    acc[atoms[0]*3+x] += grad[0][x];
+                    acc[atoms[1]*3+x] += grad[1][x];
+                    acc[atoms[2]*3+x] += grad[2][x];
+                    acc[atoms[3]*3+x] -= grad[0][x] + grad[1][x] + grad[2][x];
+                }
+            }
+        }
+#pragma omp critical
+        for (int n = 0; n < natm * 3; n++) ejk[n] += acc[n];
+        free(acc); free(bp); free(bm); free(c2a);
     }
 }
 
@@ -302,7 +472,7 @@ void oracle_pair_diag_max(const int *bas, const double *env, int nbas,
 {
 #pragma omp parallel
     {
-        double *buf = malloc(sizeof(double) * 28*28*28*28);
+        double *buf = malloc(sizeof(double) * 36*36*36*36);
 #pragma omp for schedule(dynamic, 4)
         for (int p = 0; p < npairs; p++) {
             int ish = pairs[p] / nbas, jsh = pairs[p] % nbas;
@@ -398,7 +568,7 @@ long oracle_jk(int n_dm, int nao, const double *dms, double *vj, double *vk,
     {
         double *pvj = calloc(n_dm * nao2, sizeof(double));
         double *pvk = calloc(n_dm * nao2, sizeof(double));
-        double *buf = malloc(sizeof(double) * 28*28*28*28);
+        double *buf = malloc(sizeof(double) * 36*36*36*36);
         for (int t = 0; t < ntask; t++) {
             int gi = tasks[t][0], gj = tasks[t][1], gk = tasks[t][2], gl = tasks[t][3];
             int pij0 = pair_off[gi*(gi+1)/2+gj], pij1 = pair_off[gi*(gi+1)/2+gj+1];

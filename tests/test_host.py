@@ -45,8 +45,9 @@ def test_jk_build_rejects_bad_arguments():
     a.abi_version = clib.JK_ABI_VERSION + 7
     assert lib.jk_build(ctypes.byref(a), None) == 1
     a.abi_version = clib.JK_ABI_VERSION
-    a.li, a.lj, a.lk, a.ll = 5, 0, 0, 0
+    a.li, a.lj, a.lk, a.ll = clib.JK_LMAX + 1, 0, 0, 0
     assert lib.jk_build(ctypes.byref(a), None) == 2
+    assert lib.jk_energy_ip1(ctypes.byref(a), None, 1., 1., None) == 2
     a.li = 1
     a.npairs_ij = a.npairs_kl = 3
     assert lib.jk_build(ctypes.byref(a), None) == 3
@@ -62,6 +63,15 @@ def test_jk_build_rejects_bad_arguments():
     a.n_repl, a.repl_stride = 8, 99
     assert lib.jk_build(ctypes.byref(a), None) == 3
     assert b'repl_stride' in lib.jk_last_error()
+    # the attenuation parameter of erf(omega r)/r must not be negative
+    a.n_repl, a.omega = 1, -.3
+    assert lib.jk_build(ctypes.byref(a), None) == 3
+    assert b'omega' in lib.jk_last_error()
+    # the gradient contraction takes one or two densities and an output
+    a.omega, a.n_dm = 0., 3
+    assert lib.jk_energy_ip1(ctypes.byref(a), ctypes.c_void_p(4096), 1., 1., None) == 3
+    a.n_dm = 1
+    assert lib.jk_energy_ip1(ctypes.byref(a), None, 1., 1., None) == 3
 
 
 def test_jk_build_tasks_validates_on_host():
@@ -76,7 +86,7 @@ def test_jk_build_tasks_validates_on_host():
     streams = (ctypes.c_void_p * 1)(None)
     assert lib.jk_build_tasks(ctypes.byref(a), tasks, 2, streams, 0) == 1
     assert b'jk_build_tasks' in lib.jk_last_error()
-    tasks[1].li = 5                  # launched first (last task first)
+    tasks[1].li = clib.JK_LMAX + 1   # launched first (last task first)
     assert lib.jk_build_tasks(ctypes.byref(a), tasks, 2, streams, 1) == 2
     assert b'angular class' in lib.jk_last_error()
     assert lib.jk_build_tasks(ctypes.byref(a), tasks, 0, streams, 1) == 0

@@ -208,23 +208,24 @@ def test_eri_8fold_symmetry_and_jk_identities():
     assert np.sum(d * vj) > 0
 
 
-def test_screening_drops_only_negligible():
-    '''Every quartet the predicate drops contributes below the cutoff bound.'''
+def test_q_cond_is_log_sqrt_max_diagonal():
+    """q_cond of every listed pair equals log sqrt max|(ij|ij)| of the exact
+    block (the Cauchy-Schwarz factor; the dropped quartets themselves are
+    checked by brute force in test_oracle_pins_indep.py)."""
     from gpu4pyscf_amd.gto.mole import M
     from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
     mol = M(atom='O 0 0 0; H 0 0 1.8; O 0 0 9.; H 0 1.7 9.5', unit='B', basis='sto-3g')
     sm = SortedGTO(mol)
     cache = O.build_pairs(sm, 1e-13)
-    d = np.ones((1, sm.nao, sm.nao))
-    keys = set(O.survivors(sm, d, 1, cache).tolist())
     n = sm.nbas
-    # q_cond is an upper bound of log sqrt|(ij|ij)| (Cauchy-Schwarz)
+    npairs = 0
     for (gi, gj), (p, q) in cache.items():
         for pij, qq in zip(p, q):
             i, j = divmod(int(pij), n)
             blk = O.eri_shell(sm._bas, sm.env_kernel, i, j, i, j)
             assert abs(math.log(abs(blk).max()) / 2 - qq) < 1e-5
-    assert len(keys) > 0
+            npairs += 1
+    assert npairs > 20
 
 
 def test_splitmix64_published_sequence():
