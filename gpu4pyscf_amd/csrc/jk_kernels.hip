@@ -2,6 +2,7 @@
 // the C-ABI of libjkhip.so.  The class kernels are in jk_class_kernels.h,
 // instantiated by jk_classes.hip.
 #include <stdarg.h>
+#include <mutex>
 #include "jk_class_kernels.h"
 
 static thread_local char g_err[512];
@@ -253,6 +254,41 @@ __global__ void dm_cond_log_kernel(float *cond, const double *work, int nbas, in
     cond[(size_t)ish * nbas + jsh] = (float)log(v + 1e-300);
 }
 
+// Scratch of jk_dm_cond_noalloc: the block maxima of up to
+// DM_COND_SCRATCH_NBAS^2 shell pairs (32 MiB), held by every GPU the library
+// runs on until the process ends.  Host code never passes `scratch` itself
+// to a kernel (in host code it names a host-side stand-in); it uses the
+// device address from hipGetSymbolAddress, one per device.
+#define DM_COND_SCRATCH_NBAS 2048
+static __device__ __align__(256) double scratch[DM_COND_SCRATCH_NBAS * DM_COND_SCRATCH_NBAS];
+#define MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[MAX_DEVICES];
+
+// device address of `scratch` on the current device (looked up once per
+// device); nullptr with the error set on failure
+static double *scratch_ptr(const char *who)
+{
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+        set_err("%s: no current device (%s)", who, hipGetErrorString(err));
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !ptr) {
+            set_err("%s: hipGetSymbolAddress(scratch) on device %d: %s", who, dev,
+                    hipGetErrorString(err));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (double *)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
 // ---------------------------------------------------------------- C-ABI
 static bool table_ok(const JKRysTable &t)
 {
@@ -452,6 +488,32 @@ int jk_dm_cond(float *dm_cond, double *work, const double *dm, int n_dm,
                        dm_cond, work, nbas, hermi);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) { set_err("jk_dm_cond: %s", hipGetErrorString(err)); return 4; }
+    return 0;
+}
+
+int jk_dm_cond_noalloc(float *dm_cond, const double *dm, int n_dm, int nao,
+                       const int32_t *ao_loc, int nbas, int hermi, void *stream)
+{
+    g_err[0] = 0;
+    if (!dm_cond || !dm || !ao_loc || n_dm <= 0 || nao <= 0 || nbas <= 0) {
+        set_err("jk_dm_cond_noalloc: bad arguments"); return 3;
+    }
+    if (nbas > DM_COND_SCRATCH_NBAS) {
+        set_err("jk_dm_cond_noalloc: nbas = %d exceeds the static scratch (at most %d shells)",
+                nbas, DM_COND_SCRATCH_NBAS);
+        return 3;
+    }
+    double *work = scratch_ptr("jk_dm_cond_noalloc");
+    if (!work) return 4;
+    // dm_block_max_kernel writes every one of the nbas^2 entries it reads
+    // back, so the scratch needs no clearing
+    dim3 b((nbas + 63) / 64, nbas);
+    hipLaunchKernelGGL(dm_block_max_kernel, b, dim3(64), 0, (hipStream_t)stream,
+                       work, dm, n_dm, nao, ao_loc, nbas);
+    hipLaunchKernelGGL(dm_cond_log_kernel, b, dim3(64), 0, (hipStream_t)stream,
+                       dm_cond, work, nbas, hermi);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) { set_err("jk_dm_cond_noalloc: %s", hipGetErrorString(err)); return 4; }
     return 0;
 }
 

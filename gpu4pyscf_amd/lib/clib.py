@@ -83,7 +83,7 @@ class JKTask(ctypes.Structure):
 
 
 EXPORTS = ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
-           'jk_energy_ip1',
+           'jk_dm_cond_noalloc', 'jk_energy_ip1',
            'jk_int1e', 'mlp_gemm_bf16', 'mlp_silu_backward', 'mlp_colsum', 'mlp_mse_grad',
            'mlp_adamw', 'mlp_to_bf16', 'jk_last_error', 'jk_abi_version')
 
@@ -101,7 +101,7 @@ def load():
     lib = ctypes.CDLL(LIB_PATH)
     lib.jk_last_error.restype = ctypes.c_char_p
     for name in ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
-                 'jk_energy_ip1', 'jk_abi_version'):
+                 'jk_dm_cond_noalloc', 'jk_energy_ip1', 'jk_abi_version'):
         getattr(lib, name).restype = ctypes.c_int
     lib.jk_build.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p]
     lib.jk_energy_ip1.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p, ctypes.c_double,

@@ -190,6 +190,16 @@ int jk_dm_cond(float *dm_cond, double *work, const double *dm, int n_dm,
                int nao, const int32_t *ao_loc, int nbas, int hermi,
                void *stream);
 
+/* jk_dm_cond without the work argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  The nbas x nbas block maxima live in a
+ * static device array of the library (32 MiB per GPU, held until the
+ * process ends), so nbas must not exceed 2048 (status 3 otherwise); calls
+ * no allocating routine.  Calls that overlap on one GPU share that array
+ * and race: serialise them (one stream, or events). */
+int jk_dm_cond_noalloc(float *dm_cond, const double *dm, int n_dm, int nao,
+                       const int32_t *ao_loc, int nbas, int hermi, void *stream);
+
 /* One-electron integrals on the sorted Cartesian basis (nao x nao, written
  * fully): overlap S, kinetic T and nuclear attraction
  * V = -sum_A Z_A <i|1/|r-R_A||j> of point charges.  coords: natm x 3 Bohr,
