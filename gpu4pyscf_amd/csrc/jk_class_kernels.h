@@ -476,21 +476,27 @@ __device__ __forceinline__ void load_shell_pair(ShellPair &s, uint32_t bas_ij, i
     s.i0 = ao_loc[s.ish]; s.j0 = ao_loc[s.jsh];
 }
 
-// survival predicate, create_tasks.cu:155-170, float32 in the same order
+// survival predicate, float32 in the reference's order: the bound test and
+// pair order of create_tasks.cu:150-152, then the density blocks of the
+// launch's screen (JKBuildArgs.screen): all six (:155-163), the Coulomb
+// blocks D_ij, D_kl of a J-only build (:222-225) or the exchange blocks
+// D_ik, D_jk, D_il, D_jl of a K-only build (:86-93)
 __device__ __forceinline__ bool survives(float q_kl, float kl_cutoff, float dm_penalty,
                                          uint32_t bas_ij, uint32_t bas_kl, int ish, int jsh,
-                                         int nbas, const float *dm_cond)
+                                         int nbas, const float *dm_cond, int screen)
 {
     if (!(q_kl + dm_penalty >= kl_cutoff && bas_ij >= bas_kl)) return false;
     int ksh = bas_kl / nbas;
     int lsh = bas_kl - ksh * nbas;
     float d_cutoff = kl_cutoff - q_kl;
-    return dm_cond[bas_ij] > d_cutoff ||
-           dm_cond[bas_kl] > d_cutoff ||
-           dm_cond[ish*nbas+ksh] > d_cutoff ||
-           dm_cond[jsh*nbas+ksh] > d_cutoff ||
-           dm_cond[ish*nbas+lsh] > d_cutoff ||
-           dm_cond[jsh*nbas+lsh] > d_cutoff;
+    if (screen != JK_SCREEN_K &&
+        (dm_cond[bas_ij] > d_cutoff || dm_cond[bas_kl] > d_cutoff))
+        return true;
+    return screen != JK_SCREEN_J &&
+           (dm_cond[ish*nbas+ksh] > d_cutoff ||
+            dm_cond[jsh*nbas+ksh] > d_cutoff ||
+            dm_cond[ish*nbas+lsh] > d_cutoff ||
+            dm_cond[jsh*nbas+lsh] > d_cutoff);
 }
 
 // SplitMix64 finaliser (Steele, Lea and Flood, OOPSLA 2014; constants of
@@ -503,20 +509,23 @@ __device__ __forceinline__ unsigned long long quartet_mix(uint32_t bas_ij, uint3
     return z ^ (z >> 31);
 }
 
-// Same predicate, also returning the largest of the six dm_cond values
-// (any(v > d_cutoff) == max(v) > d_cutoff).
+// Same predicate, also returning the largest dm_cond value of the screen's
+// blocks (any(v > d_cutoff) == max(v) > d_cutoff).
 __device__ __forceinline__ bool survives_dmax(float q_kl, float kl_cutoff, float dm_penalty,
                                               uint32_t bas_ij, uint32_t bas_kl, int ish, int jsh,
-                                              int nbas, const float *dm_cond, float &dmax)
+                                              int nbas, const float *dm_cond, int screen,
+                                              float &dmax)
 {
     dmax = -1e30f;
     if (!(q_kl + dm_penalty >= kl_cutoff && bas_ij >= bas_kl)) return false;
     int ksh = bas_kl / nbas;
     int lsh = bas_kl - ksh * nbas;
     float d_cutoff = kl_cutoff - q_kl;
-    dmax = fmaxf(fmaxf(fmaxf(dm_cond[bas_ij], dm_cond[bas_kl]),
-                       fmaxf(dm_cond[ish*nbas+ksh], dm_cond[jsh*nbas+ksh])),
-                 fmaxf(dm_cond[ish*nbas+lsh], dm_cond[jsh*nbas+lsh]));
+    if (screen != JK_SCREEN_K)
+        dmax = fmaxf(dm_cond[bas_ij], dm_cond[bas_kl]);
+    if (screen != JK_SCREEN_J)
+        dmax = fmaxf(dmax, fmaxf(fmaxf(dm_cond[ish*nbas+ksh], dm_cond[jsh*nbas+ksh]),
+                                 fmaxf(dm_cond[ish*nbas+lsh], dm_cond[jsh*nbas+lsh])));
     return dmax > d_cutoff;
 }
 
@@ -1017,7 +1026,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                 float dmax;
                 const uint32_t bkl = a.pairs_kl[pkl];
                 keep = survives_dmax(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
-                                     sp.ish, sp.jsh, nbas, a.dm_cond, dmax);
+                                     sp.ish, sp.jsh, nbas, a.dm_cond, a.screen, dmax);
                 lowp = keep && (q_ij + q_kl + dmax < a.fp32_cut);
                 if (a.quartet_hash && keep) hsum += quartet_mix(bas_ij, bkl);
             }
@@ -1164,7 +1173,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                 bkl = a.pairs_kl[pkl];
                 fail_margin = q_kl + a.dm_penalty + Q_COND_MARGIN < kl_cutoff;
                 keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
-                                sp.ish, sp.jsh, nbas, a.dm_cond);
+                                sp.ish, sp.jsh, nbas, a.dm_cond, a.screen);
                 if (a.quartet_hash && keep) hsum += quartet_mix(bas_ij, bkl);
             }
             uint64_t m = __ballot(keep);

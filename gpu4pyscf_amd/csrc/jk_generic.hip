@@ -187,7 +187,7 @@ void jk_generic_kernel(JKBuildArgs a, int kl_chunk, GenExtra ex)
                 fail_margin = q_kl + penalty + Q_COND_MARGIN < kl_cutoff;
                 if (MODE == GEN_JK)
                     keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl, sp.ish, sp.jsh,
-                                    nbas, a.dm_cond);
+                                    nbas, a.dm_cond, a.screen);
                 else
                     keep = survives_ejk(q_kl, kl_cutoff, bas_ij, bkl, sp.ish, sp.jsh, nbas,
                                         a.dm_cond, do_j, do_k);

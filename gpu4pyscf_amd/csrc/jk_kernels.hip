@@ -371,6 +371,12 @@ int jk_build(const JKBuildArgs *args, void *stream)
         if (st) return st;
     }
     if (!a.vj && !a.vk) { set_err("jk_build: neither vj nor vk given"); return 3; }
+    if (a.screen < JK_SCREEN_JK || a.screen > JK_SCREEN_K ||
+        (a.screen == JK_SCREEN_J && a.vk) || (a.screen == JK_SCREEN_K && a.vj)) {
+        set_err("jk_build: screen %d does not match the requested matrices (J-only screen "
+                "takes no vk, K-only no vj)", a.screen);
+        return 3;
+    }
     if (a.n_repl > 1 && a.repl_stride < (int64_t)a.n_dm * a.nao * a.nao) {
         set_err("jk_build: repl_stride smaller than %d x nao^2", a.n_dm);
         return 3;

@@ -1,13 +1,12 @@
 // BF16 MFMA kernels of the learned initial-guess network: GEMM with fused
 // bias + SiLU epilogue, SiLU backward, masked MSE gradient and AdamW.
 // Weights and activations are bfloat16, every accumulation is FP32
-// (v_mfma_f32_16x16x16_bf16: one wavefront computes a 16x16 FP32 tile).
+// (v_mfma_f32_16x16x32_bf16, LDS-staged 64 x 64 workgroup tiles).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
 #include "../../include/jkhip.h"
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 void jk_set_error(const char *msg);
@@ -32,38 +31,95 @@ __device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); 
 // C(m, n) = sum_k A(m, k) B(k, n), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn].
 // mode 0: C = acc + bias;  mode 1: C = Z = acc + bias and act = bf16(silu(Z));
 // mode 2: C += acc.
-// One wavefront per 16x16 tile.  Fragment layout of the 16x16x16 bf16 MFMA:
-// lane l supplies A(l%16, 4(l/16)+0..3) and B(4(l/16)+0..3, l%16) and
-// receives D(4(l/16)+0..3, l%16).
-__global__ __launch_bounds__(64)
+// Workgroup tile GM x GN (64 x 64) over 4 wavefronts, each a 32 x 32 quarter
+// as 2 x 2 tiles of v_mfma_f32_16x16x32_bf16; K advances GK = 32 at a time.
+// Both operand tiles are staged in LDS with k contiguous (A as [m][k], B
+// transposed as [n][k]), loaded from HBM along whichever of the operand's
+// two dimensions has unit stride, so that forward (A row major), dW = X^T dZ
+// (A = X^T: unit stride along m) and dA = dZ W^T (B = W^T: unit stride
+// along k) all read coalesced.  Fragment layout of the 16x16x32 bf16 MFMA:
+// lane l supplies A(l%16, 8(l/16)+0..7) and B(8(l/16)+0..7, l%16) (16 bytes
+// of one LDS row each) and receives D(4(l/16)+0..3, l%16).
+#define GM 64
+#define GN 64
+#define GK 32
+#define GK_PAD 40           // LDS row stride in bf16 (80 bytes: 16-byte aligned, spreads banks)
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+
+// one GR x GK operand tile into LDS t[r][k] (rows r of the tile = m of A or n of
+// B); element (r, k) sits at P[r*sr + k*sk]; zero outside R x K
+template <int GR>
+__device__ __forceinline__ void stage_tile(uint16_t (*t)[GK_PAD], const uint16_t *P, long sr,
+                                           long sk, int r0, int k0, int R, int K, int tid)
+{
+    constexpr int PER = GR * GK / 256;
+    if (sk == 1) {
+        // consecutive threads along k (rows of 32 elements)
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int n = tid + e * 256;
+            const int r = n / GK, k = n % GK;
+            const int gr = r0 + r, gk = k0 + k;
+            t[r][k] = (gr < R && gk < K) ? P[gr * sr + gk] : (uint16_t)0;
+        }
+    } else {
+        // consecutive threads along r
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int n = tid + e * 256;
+            const int k = n / GR, r = n % GR;
+            const int gr = r0 + r, gk = k0 + k;
+            t[r][k] = (gr < R && gk < K) ? P[gr * sr + gk * sk] : (uint16_t)0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256)
 void gemm_bf16_kernel(float *C, uint16_t *act, const uint16_t *A, const uint16_t *B,
                       const float *bias, int M, int N, int K, long sam, long sak,
                       long sbk, long sbn, int ldc, int mode)
 {
-    const int lane = threadIdx.x;
-    const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
+    __shared__ __attribute__((aligned(16))) uint16_t As[GM][GK_PAD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[GN][GK_PAD];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int m0 = blockIdx.x * GM, n0 = blockIdx.y * GN;
+    const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
     const int r = lane & 15, g = lane >> 4;
-    const int am = m0 + r;
-    const int bn = n0 + r;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x4 av, bv;
+    f32x4 acc[2][2];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            int k = k0 + 4 * g + t;
-            uint16_t x = 0, y = 0;
-            if (am < M && k < K) x = A[am * sam + k * sak];
-            if (bn < N && k < K) y = B[k * sbk + bn * sbn];
-            av[t] = (short)x;
-            bv[t] = (short)y;
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        __syncthreads();
+        stage_tile<GM>(As, A, sam, sak, m0, k0, M, K, tid);
+        stage_tile<GN>(Bs, B, sbn, sbk, n0, k0, N, K, tid);
+        __syncthreads();
+        bf16x8 av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+            av[a] = *reinterpret_cast<const bf16x8 *>(&As[wm + 16 * a + r][8 * g]);
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            bv[b] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + 16 * b + r][8 * g]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b],
+                                                                    0, 0, 0);
     }
 #pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int m = m0 + 4 * g + t, n = n0 + r;
+        const int m = m0 + wm + 16 * a + 4 * g + t, n = n0 + wn + 16 * b + r;
         if (m >= M || n >= N) continue;
-        float v = acc[t];
+        float v = acc[a][b][t];
         const long idx = (long)m * ldc + n;
         if (mode == 2) {
             C[idx] += v;
@@ -169,9 +225,9 @@ int mlp_gemm_bf16(float *C, uint16_t *act, const uint16_t *A, const uint16_t *B,
         jk_set_error("mlp_gemm_bf16: bad arguments");
         return 3;
     }
-    dim3 grid((M + 15) / 16, (N + 15) / 16);
+    dim3 grid((M + GM - 1) / GM, (N + GN - 1) / GN);
     if (grid.y > 65535u) { jk_set_error("mlp_gemm_bf16: N too large"); return 3; }
-    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(64), 0, (hipStream_t)stream, C, act, A, B,
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, C, act, A, B,
                        bias, M, N, K, sam, sak, sbk, sbn, ldc, mode);
     return finish("mlp_gemm_bf16");
 }

@@ -24,13 +24,29 @@ from gpu4pyscf_amd.gto.mole import (ANG_OF, NPRIM_OF, NCTR_OF, PTR_EXP, PTR_COEF
 from gpu4pyscf_amd.gto.cart2sph import c2s
 
 
-def _split_shell(l, exps, cs, diffuse_cutoff):
+def _uncontract(l, exps, cs):
+    '''Every primitive its own normalised shell; recon[ictr, p] is the
+    weight of primitive p in contracted function ictr.'''
+    nprim = len(exps)
+    norm = gto_norm(l, exps)
+    cn = cs.T / norm
+    max_c = abs(cn).max(axis=0) + 1e-200
+    cn = cn / max_c
+    pieces = [(exps[p:p+1], np.array([norm[p] * max_c[p]])) for p in range(nprim)]
+    return pieces, cn
+
+
+def _split_shell(l, exps, cs, diffuse_cutoff, decontract=False):
     '''Split one shell (exps[nprim], cs[nprim, nctr]) into segment shells.
 
     Returns ``pieces`` = [(exps, coeffs)], and ``recon`` with
     recon[ictr, ipiece] = coefficient of piece ipiece in contracted function
-    ictr (gpu4pyscf/gto/mole.py:1157-1251).'''
+    ictr (gpu4pyscf/gto/mole.py:1157-1251).  decontract: one shell per
+    primitive (the basis of the MD J-engine, SortedGTO.from_mol(...,
+    decontract=True) in the reference).'''
     nprim, nctr = cs.shape
+    if decontract:
+        return _uncontract(l, exps, cs)
     if nctr == 1:
         c = cs[:, 0]
         if diffuse_cutoff == 0 or np.all(exps > diffuse_cutoff):
@@ -51,12 +67,7 @@ def _split_shell(l, exps, cs, diffuse_cutoff):
     seg = np.where(nprim_each > 1)[0]
     if len(seg) != 1:
         # full uncontraction; every primitive is its own shell
-        norm = gto_norm(l, exps)
-        cn = c / norm
-        max_c = abs(cn).max(axis=0) + 1e-200
-        cn = cn / max_c
-        pieces = [(exps[p:p+1], np.array([norm[p] * max_c[p]])) for p in range(nprim)]
-        return pieces, cn
+        return _uncontract(l, exps, cs)
     seg = seg[0]
     rem = np.where(np.arange(nctr) != seg)[0]
     diffuse_mask = np.any(c[rem] != 0, axis=0)
@@ -75,8 +86,9 @@ def _split_shell(l, exps, cs, diffuse_cutoff):
 class SortedGTO:
     '''Sorted Cartesian basis derived from a PySCF-layout molecule.'''
 
-    def __init__(self, mol, diffuse_cutoff=0.3):
+    def __init__(self, mol, diffuse_cutoff=0.3, decontract=False):
         self.mol = mol
+        self.decontract = decontract
         atm = np.asarray(mol._atm, dtype=np.int32)
         bas0 = np.asarray(mol._bas)
         env0 = np.asarray(mol._env, dtype=np.float64)
@@ -88,7 +100,7 @@ class SortedGTO:
             l = int(b[ANG_OF]); nprim = int(b[NPRIM_OF]); nctr = int(b[NCTR_OF])
             exps = env0[b[PTR_EXP]:b[PTR_EXP]+nprim].copy()
             cs = env0[b[PTR_COEFF]:b[PTR_COEFF]+nprim*nctr].reshape(nctr, nprim).T.copy()
-            pieces, recon = _split_shell(l, exps, cs, diffuse_cutoff)
+            pieces, recon = _split_shell(l, exps, cs, diffuse_cutoff, decontract)
             ids = []
             for e, c in pieces:
                 ids.append(len(shells))

@@ -14,10 +14,12 @@ _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, 'libjkhip.so')
 TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
 
-JK_ABI_VERSION = 9
+JK_ABI_VERSION = 10
 JK_RYS_NMAX = 11
 JK_BOYS_M = 13
 JK_LMAX = 5
+# JKBuildArgs.screen
+JK_SCREEN = {'jk': 0, 'j': 1, 'k': 2}
 
 c_void_p = ctypes.c_void_p
 
@@ -71,6 +73,7 @@ class JKBuildArgs(ctypes.Structure):
         ('repl_stride', ctypes.c_int64),
         ('omega', ctypes.c_double),
         ('scale', ctypes.c_double),
+        ('screen', ctypes.c_int),
     ]
 
 

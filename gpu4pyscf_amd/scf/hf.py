@@ -121,6 +121,11 @@ class RHF:
     # accumulate per-phase wall times (jk, eig, diis) in self.timings; the
     # device synchronisations this needs cost a little time
     time_phases = False
+    # J and K of every build: one J/K build screened on all six density
+    # blocks (False), or, as the reference's RHF does
+    # (gpu4pyscf/scf/hf.py:91-109), a J-only build and a K-only build, each
+    # screened on its own blocks (True)
+    split_jk = False
 
     def __init__(self, mol):
         self.mol = mol
@@ -150,15 +155,16 @@ class RHF:
         S, T, V = int1e(self.mol, self.opt)
         return T + V, S
 
-    def get_jk(self, mol=None, dm=None, hermi=1, with_j=True, with_k=True):
-        '''J and K of ``dm`` (device tensor in the original AO basis).'''
+    def get_jk(self, mol=None, dm=None, hermi=1, with_j=True, with_k=True, screen=None):
+        '''J and K of ``dm`` (device tensor in the original AO basis); screen
+        as for _VHFOpt.get_jk.'''
         self.jk_builds += 1
         if self.jk_hook is not None:
             return self.jk_hook(dm, hermi)
         opt = self.opt
         dms = opt.apply_coeff_C_mat_CT(dm.reshape(-1, dm.shape[-1], dm.shape[-1]))
         vj, vk = opt.get_jk(dms, hermi, with_j=with_j, with_k=with_k,
-                            fp32_tol=self._fp32_now, count_quartets=True)
+                            fp32_tol=self._fp32_now, count_quartets=True, screen=screen)
         self.quartets += opt.last_quartets
         self.quartets_fp32 += opt.last_quartets_fp32
         shape = dm.shape
@@ -168,11 +174,15 @@ class RHF:
 
     def get_veff(self, dm, dm_last=None, vhf_last=None):
         '''J - K/2 built on the density change (gpu4pyscf/scf/hf.py:91-109).'''
-        if dm_last is None or vhf_last is None:
-            vj, vk = self.get_jk(self.mol, dm, hermi=1)
-            return vj - .5 * vk
-        vj, vk = self.get_jk(self.mol, dm - dm_last, hermi=1)
-        return vhf_last + vj - .5 * vk
+        incremental = dm_last is not None and vhf_last is not None
+        ddm = dm - dm_last if incremental else dm
+        if self.split_jk:
+            vj = self.get_jk(self.mol, ddm, hermi=1, with_k=False, screen='j')[0]
+            vk = self.get_jk(self.mol, ddm, hermi=1, with_j=False, screen='k')[1]
+        else:
+            vj, vk = self.get_jk(self.mol, ddm, hermi=1)
+        v = vj - .5 * vk
+        return vhf_last + v if incremental else v
 
     # -- linear algebra ---------------------------------------------------
     def _orth(self, S):

@@ -320,7 +320,8 @@ class _VHFOpt:
 
     def get_jk(self, dms, hermi, verbose=None, with_j=True, with_k=True,
                count_quartets=False, time_classes=False, fp32_tol=None,
-               hash_quartets=False, omega=None, lr_factor=None, sr_factor=None):
+               hash_quartets=False, omega=None, lr_factor=None, sr_factor=None,
+               screen=None):
         """J/K for density matrices ``dms`` (n_dm, nao_c, nao_c) in the sorted
         Cartesian basis (torch, float64, on the device).
 
@@ -333,15 +334,37 @@ class _VHFOpt:
         with HIP events on the stream it runs on, in ``self.last_task_ms``.
         fp32_tol: evaluate in FP32 the quartets whose bound
         sqrt|(ij|ij)| sqrt|(kl|kl)| max|D| is below fp32_tol (None: all FP64).
-        omega, lr_factor, sr_factor: the operator of J and K (default: the
+        omega, lr_factor, sr_factor: the operator of K (default: the
         molecule's omega); a range-separated operator takes two launches of
         every class task (full range and erf-attenuated), whose counters are
-        those of the first."""
+        those of the first.  J always uses the molecule's operator; when K's
+        differs, J comes from a J-only build and K from a K-only build, as in
+        gpu4pyscf/scf/jk.py:129-135 (the counters are then the K build's).
+        screen: the density blocks of the survival test, 'jk' (all six, the
+        default of a J/K build, RYS_build_jk), 'j' (D_ij, D_kl: a J-only
+        build, RYS_build_j, whose densities are first symmetrised as in
+        gpu4pyscf/scf/jk.py:224-226) or 'k' (D_ik, D_jk, D_il, D_jl: a
+        K-only build, RYS_build_k)."""
         torch = _torch()
         if self._dev is None:
             self.build()
         if callable(dms):
             dms = dms()
+        if (screen is None and omega is not None and with_j and with_k and
+                self.operator_passes() != self.operator_passes(omega, lr_factor, sr_factor)):
+            kw = dict(count_quartets=count_quartets, time_classes=time_classes,
+                      fp32_tol=fp32_tol, hash_quartets=hash_quartets)
+            vj = self.get_jk(dms, hermi, verbose, True, False, screen='j', **kw)[0]
+            vk = self.get_jk(dms, hermi, verbose, False, True, omega=omega,
+                             lr_factor=lr_factor, sr_factor=sr_factor, screen='k', **kw)[1]
+            return vj, vk
+        screen = screen or 'jk'
+        if screen not in clib.JK_SCREEN or (screen == 'j' and with_k) or \
+                (screen == 'k' and with_j):
+            raise ValueError(f'screen {screen!r} with with_j={with_j}, with_k={with_k}')
+        if screen == 'j' and hermi != 1:
+            dms = (dms + dms.transpose(1, 2)) * .5
+            hermi = 1
         lib = clib.load()
         mol = self.sorted_mol
         d = self._dev
@@ -394,21 +417,14 @@ class _VHFOpt:
         args.fp32_cut = np.float32(math.log(fp32_tol)) if fp32_tol else np.float32(-1e30)
         args.n_repl = n_repl
         args.repl_stride = dms.numel()
+        args.screen = clib.JK_SCREEN[screen]
         uniq = mol.uniq_l_ctr
         goff = mol.group_offsets
         g_ao0 = mol.ao_loc[goff[:-1]]
         g_nao = mol.ao_loc[goff[1:]] - g_ao0
-        # launch passes: (omega, scale, with J, with K); J and K share the
-        # launches when they use the same operator
-        passes = self.operator_passes(omega, lr_factor, sr_factor)
-        if omega is not None and with_j and with_k:
-            pj = self.operator_passes()
-            if pj != passes:
-                runs = [(w, f, True, False) for w, f in pj] + [(w, f, False, True) for w, f in passes]
-            else:
-                runs = [(w, f, True, True) for w, f in passes]
-        else:
-            runs = [(w, f, with_j, with_k) for w, f in passes]
+        # launch passes of the operator: (omega, scale, with J, with K)
+        runs = [(w, f, with_j, with_k)
+                for w, f in self.operator_passes(omega, lr_factor, sr_factor)]
         cur = torch.cuda.current_stream(dms.device)
         streams = [cur]
         if self.nstreams > 1:
@@ -570,7 +586,13 @@ def get_jk(mol, dm, hermi=0, vhfopt=None, with_j=True, with_k=True,
     Returns (vj, vk) with the shape of ``dm``; an element is None when not
     requested.  With omega (and lr_factor, sr_factor) given, K uses the
     operator lr_factor erf(|omega| r)/r + sr_factor erfc(|omega| r)/r and J
-    the molecule's operator (mol.omega), as in the reference."""
+    the molecule's operator (mol.omega), as in the reference.  The quartets
+    are screened on all six density blocks (the reference's RYS_build_jk)
+    whichever matrices are requested."""
+    return _get_jk(mol, dm, hermi, vhfopt, with_j, with_k, omega, lr_factor, sr_factor, None)
+
+
+def _get_jk(mol, dm, hermi, vhfopt, with_j, with_k, omega, lr_factor, sr_factor, screen):
     assert with_j or with_k
     if vhfopt is None:
         vhfopt = _VHFOpt(mol).build()
@@ -579,8 +601,8 @@ def get_jk(mol, dm, hermi=0, vhfopt=None, with_j=True, with_k=True,
     nao0 = shape[-1]
     dms = dm_t.reshape(-1, nao0, nao0)
     dms = vhfopt.apply_coeff_C_mat_CT(dms)
-    vj, vk = vhfopt.get_jk(dms, hermi, verbose, with_j=with_j, with_k=with_k,
-                           omega=omega, lr_factor=lr_factor, sr_factor=sr_factor)
+    vj, vk = vhfopt.get_jk(dms, hermi, with_j=with_j, with_k=with_k,
+                           omega=omega, lr_factor=lr_factor, sr_factor=sr_factor, screen=screen)
     if with_j:
         vj = vhfopt.apply_coeff_CT_mat_C(vj).reshape(shape)
     if with_k:
@@ -592,11 +614,12 @@ def get_jk(mol, dm, hermi=0, vhfopt=None, with_j=True, with_k=True,
 
 
 def get_j(mol, dm, hermi=0, vhfopt=None, verbose=None):
-    '''Coulomb matrix, gpu4pyscf/scf/jk.py:206-230.'''
-    return get_jk(mol, dm, hermi, vhfopt, with_j=True, with_k=False)[0]
+    '''Coulomb matrix by a J-only build (screened on D_ij and D_kl),
+    gpu4pyscf/scf/jk.py:206-230 and its _VHFOpt.get_j (:643-783).'''
+    return _get_jk(mol, dm, hermi, vhfopt, True, False, None, None, None, 'j')[0]
 
 
 def get_k(mol, dm, hermi=0, vhfopt=None, omega=None, lr_factor=None, sr_factor=None):
-    '''Exchange matrix, gpu4pyscf/scf/jk.py:153-204.'''
-    return get_jk(mol, dm, hermi, vhfopt, with_j=False, with_k=True, omega=omega,
-                  lr_factor=lr_factor, sr_factor=sr_factor)[1]
+    '''Exchange matrix by a K-only build (screened on D_ik, D_jk, D_il and
+    D_jl), gpu4pyscf/scf/jk.py:153-204 and its _VHFOpt.get_k (:785-927).'''
+    return _get_jk(mol, dm, hermi, vhfopt, False, True, omega, lr_factor, sr_factor, 'k')[1]

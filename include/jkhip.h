@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  9
+#define JK_ABI_VERSION  10
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         5       /* max angular momentum on the GPU path */
 
@@ -109,7 +109,18 @@ typedef struct {
      * doubles the Rys roots (rys_roots_for_k.cu:19-68). */
     double omega;
     double scale;           /* 0 is taken as 1 */
+    /* Survival screen: which density blocks keep a quartet whose bound
+     * q_ij + q_kl + dm_cond exceeds the cutoff.  JK_SCREEN_JK: any of the
+     * six blocks D_ij, D_kl, D_ik, D_jk, D_il, D_jl (RYS_build_jk,
+     * create_tasks.cu:155-163); JK_SCREEN_J: D_ij or D_kl, vk must be NULL
+     * (RYS_build_j, create_tasks.cu:222-225); JK_SCREEN_K: D_ik, D_jk, D_il
+     * or D_jl, vj must be NULL (RYS_build_k, create_tasks.cu:86-93). */
+    int screen;
 } JKBuildArgs;
+
+#define JK_SCREEN_JK    0
+#define JK_SCREEN_J     1
+#define JK_SCREEN_K     2
 
 /* Accumulate one class task into vj/vk (before the hermi post-processing of
  * gpu4pyscf/scf/jk.py:594-616).  stream: hipStream_t or NULL. */

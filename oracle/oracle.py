@@ -44,6 +44,7 @@ def lib():
         _lib.oracle_splitmix64_mix.restype = ctypes.c_uint64
         _lib.oracle_splitmix64_mix.argtypes = [ctypes.c_uint64]
         _lib.oracle_set_operator.argtypes = [ctypes.c_double] * 3
+        _lib.oracle_set_screen.argtypes = [ctypes.c_int]
         _lib.oracle_ejk_ip1.argtypes = ([ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 2 +
                                         [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int] +
                                         [ctypes.c_double] * 2 + [ctypes.c_void_p])
@@ -74,6 +75,24 @@ def set_operator(omega=0., lr_factor=None, sr_factor=None):
 
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# survival screens: the six density blocks of a J/K build, the two Coulomb
+# blocks of a J-only build, the four exchange blocks of a K-only build
+# (gpu4pyscf/lib/gvhf-rys/create_tasks.cu:155-163, 222-225, 86-93)
+SCREENS = {'jk': 0, 'j': 1, 'k': 2}
+
+
+class _screen:
+    '''Context: the C oracle's survival screen.'''
+    def __init__(self, screen):
+        self.code = SCREENS[screen]
+
+    def __enter__(self):
+        lib().oracle_set_screen(self.code)
+
+    def __exit__(self, *exc):
+        lib().oracle_set_screen(0)
 
 
 def rys_roots(n, x):
@@ -221,10 +240,16 @@ def flatten_pairs(cache, ngroups):
 
 
 def jk_sorted(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13,
-              with_j=True, with_k=True, list_quartets=0, ij_stride=1):
+              with_j=True, with_k=True, list_quartets=0, ij_stride=1, screen='jk'):
     '''J/K on the sorted Cartesian basis including the hermi post-processing.
-    Returns (vj, vk, n_surviving, quartet_list or None).'''
+    Returns (vj, vk, n_surviving, quartet_list or None).  screen 'j' (a
+    J-only build) first symmetrises the densities as the reference's get_j
+    does (gpu4pyscf/scf/jk.py:224-226), 'k' is a K-only build
+    (gpu4pyscf/scf/jk.py:785-927).'''
     dms = np.asarray(dms, dtype=np.float64)
+    if screen == 'j' and hermi != 1:
+        dms = (dms + dms.transpose(0, 2, 1)) * .5
+        hermi = 1
     ao_loc = sorted_mol.ao_loc
     cond, penalty = dm_condition(dms, ao_loc, hermi)
     if hermi == 0:
@@ -247,13 +272,14 @@ def jk_sorted(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13,
         quartets = np.zeros((list_quartets, 2), dtype=np.uint32)
         qptr = _p(quartets)
     L = lib()
-    nsurv = L.oracle_jk(
-        ctypes.c_int(n_dm), ctypes.c_int(nao), _p(dms_run), _p(vj), _p(vk),
-        _p(bas), ctypes.c_int(sorted_mol.nbas), _p(env), _p(aol),
-        ctypes.c_int(ng), _p(off), _p(pairs), _p(q), _p(cond),
-        ctypes.c_float(cutoff), ctypes.c_float(penalty),
-        ctypes.c_int(int(with_j)), ctypes.c_int(int(with_k)),
-        qptr, ctypes.c_long(list_quartets), ctypes.c_int(ij_stride))
+    with _screen(screen):
+        nsurv = L.oracle_jk(
+            ctypes.c_int(n_dm), ctypes.c_int(nao), _p(dms_run), _p(vj), _p(vk),
+            _p(bas), ctypes.c_int(sorted_mol.nbas), _p(env), _p(aol),
+            ctypes.c_int(ng), _p(off), _p(pairs), _p(q), _p(cond),
+            ctypes.c_float(cutoff), ctypes.c_float(penalty),
+            ctypes.c_int(int(with_j)), ctypes.c_int(int(with_k)),
+            qptr, ctypes.c_long(list_quartets), ctypes.c_int(ij_stride))
     if hermi == 1:
         vj *= 2.
         vk = vk + vk.transpose(0, 2, 1)
@@ -268,9 +294,12 @@ def jk_sorted(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13,
     return vj, vk, nsurv, quartets
 
 
-def survivors(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13):
+def survivors(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13, screen='jk'):
     '''Sorted array of surviving quartet keys bas_ij*nbas^2 + bas_kl.'''
     dms = np.asarray(dms, dtype=np.float64)
+    if screen == 'j' and hermi != 1:
+        dms = (dms + dms.transpose(0, 2, 1)) * .5
+        hermi = 1
     cond, penalty = dm_condition(dms, sorted_mol.ao_loc, hermi)
     ng = len(sorted_mol.l_ctr_counts)
     off, pairs, q = flatten_pairs(cache, ng)
@@ -282,12 +311,13 @@ def survivors(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13):
     cap = 1
     while True:
         out = np.zeros((cap, 2), dtype=np.uint32)
-        n = L.oracle_jk(ctypes.c_int(0), ctypes.c_int(sorted_mol.nao), None, None, None,
+        with _screen(screen):
+            n = L.oracle_jk(ctypes.c_int(0), ctypes.c_int(sorted_mol.nao), None, None, None,
                         _p(bas), ctypes.c_int(sorted_mol.nbas), _p(env), _p(aol),
                         ctypes.c_int(ng), _p(off), _p(pairs), _p(q), _p(cond),
                         ctypes.c_float(cutoff), ctypes.c_float(penalty),
-                        ctypes.c_int(0), ctypes.c_int(0), _p(out), ctypes.c_long(cap),
-                        ctypes.c_int(1))
+                            ctypes.c_int(0), ctypes.c_int(0), _p(out), ctypes.c_long(cap),
+                            ctypes.c_int(1))
         if n <= cap:
             out = out[:n]
             break
@@ -297,11 +327,14 @@ def survivors(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13):
     return np.sort(keys)
 
 
-def quartet_sets(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13):
+def quartet_sets(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13, screen='jk'):
     '''{(i, j, k, l): (count, checksum)} of the surviving quartets of every
     class task with survivors: checksum = sum mod 2**64 of the SplitMix64
     finaliser of bas_ij << 32 | bas_kl (the C oracle's quartet_mix).'''
     dms = np.asarray(dms, dtype=np.float64)
+    if screen == 'j' and hermi != 1:
+        dms = (dms + dms.transpose(0, 2, 1)) * .5
+        hermi = 1
     cond, penalty = dm_condition(dms, sorted_mol.ao_loc, hermi)
     ng = len(sorted_mol.l_ctr_counts)
     off, pairs, q = flatten_pairs(cache, ng)
@@ -309,10 +342,11 @@ def quartet_sets(sorted_mol, dms, hermi, cache, direct_scf_tol=1e-13):
              for k in range(i + 1) for l in range(k + 1)]
     counts = np.zeros(len(tasks), dtype=np.int64)
     hashes = np.zeros(len(tasks), dtype=np.uint64)
-    lib().oracle_quartet_sets(
-        ctypes.c_int(sorted_mol.nbas), ctypes.c_int(ng), _p(off), _p(pairs), _p(q),
-        _p(cond), ctypes.c_float(np.float32(math.log(direct_scf_tol))),
-        ctypes.c_float(penalty), _p(counts), _p(hashes))
+    with _screen(screen):
+        lib().oracle_quartet_sets(
+            ctypes.c_int(sorted_mol.nbas), ctypes.c_int(ng), _p(off), _p(pairs), _p(q),
+            _p(cond), ctypes.c_float(np.float32(math.log(direct_scf_tol))),
+            ctypes.c_float(penalty), _p(counts), _p(hashes))
     return {t: (int(c), int(h)) for t, c, h in zip(tasks, counts, hashes) if c}
 
 
@@ -322,10 +356,14 @@ def splitmix64_mix(z):
 
 
 def get_jk(mol, dm, hermi=1, with_j=True, with_k=True, direct_scf_tol=1e-13,
-           sorted_mol=None, cache=None, omega=None, lr_factor=None, sr_factor=None):
+           sorted_mol=None, cache=None, omega=None, lr_factor=None, sr_factor=None,
+           screen='jk'):
     """Oracle J/K in the original AO basis of ``mol``.  The screening uses
     the molecule's operator (mol.omega); K uses (omega, lr_factor,
-    sr_factor) when omega is given, J always the molecule's operator."""
+    sr_factor) when omega is given, J always the molecule's operator.
+    screen: 'jk' (the six density blocks of gpu4pyscf's get_jk), 'j' (J only,
+    get_j), 'k' (K only, get_k); with a range-separated K the two halves use
+    the 'j' and 'k' screens, as gpu4pyscf/scf/jk.py:129-135 does."""
     from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
     mol_omega = getattr(mol, 'omega', 0.) or 0.
     if sorted_mol is None:
@@ -342,14 +380,16 @@ def get_jk(mol, dm, hermi=1, with_j=True, with_k=True, direct_scf_tol=1e-13,
         dms_c = np.einsum('pi,nij,qj->npq', C, dms, C)
         if omega is None or rsh_factors(omega, lr_factor, sr_factor) == rsh_factors(mol_omega):
             vj, vk, nsurv, _ = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol,
-                                         with_j, with_k)
+                                         with_j, with_k, screen=screen)
         else:
             vj = vk = None
             if with_j:
-                vj = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol, True, False)[0]
+                vj = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol, True, False,
+                               screen='j')[0]
             if with_k:
                 set_operator(omega, lr_factor, sr_factor)
-                vk = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol, False, True)[1]
+                vk = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol, False, True,
+                               screen='k')[1]
     finally:
         set_operator(0.)
     vj = np.einsum('pi,npq,qj->nij', C, vj, C).reshape(shape) if with_j else None

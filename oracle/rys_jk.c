@@ -489,6 +489,15 @@ void oracle_pair_diag_max(const int *bas, const double *env, int nbas,
 }
 
 /* Reference predicate (create_tasks.cu:155-170), float32 arithmetic. */
+/* Which density blocks decide survival (oracle_set_screen): 0 the six
+ * blocks of a J/K build (create_tasks.cu:155-163, _fill_vjk_tasks), 1 the
+ * two Coulomb blocks D_ij, D_kl of a J-only build (create_tasks.cu:222-225,
+ * _fill_vj_tasks), 2 the four exchange blocks D_ik, D_jk, D_il, D_jl of a
+ * K-only build (create_tasks.cu:86-93, _fill_vk_tasks). */
+static int g_screen = 0;
+
+void oracle_set_screen(int screen) { g_screen = screen; }
+
 static inline int survives(float q_ij, float q_kl, float cutoff, float dm_penalty,
                            uint32_t bas_ij, uint32_t bas_kl,
                            int ish, int jsh, int ksh, int lsh, int nbas,
@@ -498,12 +507,10 @@ static inline int survives(float q_ij, float q_kl, float cutoff, float dm_penalt
     int keep = q_kl + dm_penalty >= kl_cutoff && bas_ij >= bas_kl;
     if (!keep) return 0;
     float d_cutoff = kl_cutoff - q_kl;
-    return (dm_cond[bas_ij] > d_cutoff ||
-            dm_cond[bas_kl] > d_cutoff ||
-            dm_cond[ish*nbas+ksh] > d_cutoff ||
-            dm_cond[jsh*nbas+ksh] > d_cutoff ||
-            dm_cond[ish*nbas+lsh] > d_cutoff ||
-            dm_cond[jsh*nbas+lsh] > d_cutoff);
+    int coulomb = dm_cond[bas_ij] > d_cutoff || dm_cond[bas_kl] > d_cutoff;
+    int exchange = dm_cond[ish*nbas+ksh] > d_cutoff || dm_cond[jsh*nbas+ksh] > d_cutoff ||
+                   dm_cond[ish*nbas+lsh] > d_cutoff || dm_cond[jsh*nbas+lsh] > d_cutoff;
+    return g_screen == 1 ? coulomb : (g_screen == 2 ? exchange : (coulomb || exchange));
 }
 
 /*

@@ -63,6 +63,19 @@ def test_jk_build_rejects_bad_arguments():
     a.n_repl, a.repl_stride = 8, 99
     assert lib.jk_build(ctypes.byref(a), None) == 3
     assert b'repl_stride' in lib.jk_last_error()
+    # a J-only screen builds no K, a K-only screen no J
+    a.n_repl = 1
+    for screen, keep in ((clib.JK_SCREEN['j'], 'vj'), (clib.JK_SCREEN['k'], 'vk'), (7, None)):
+        a.screen = screen
+        assert lib.jk_build(ctypes.byref(a), None) == 3
+        assert b'screen' in lib.jk_last_error()
+        if keep:
+            saved = (a.vj, a.vk)
+            a.vj, a.vk = (4096, None) if keep == 'vj' else (None, 4096)
+            st = lib.jk_build(ctypes.byref(a), None)
+            a.vj, a.vk = saved
+            assert b'screen' not in lib.jk_last_error() or st == 0
+    a.screen = 0
     # the attenuation parameter of erf(omega r)/r must not be negative
     a.n_repl, a.omega = 1, -.3
     assert lib.jk_build(ctypes.byref(a), None) == 3

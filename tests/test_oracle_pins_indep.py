@@ -174,15 +174,22 @@ def _block_max(D, ao_loc, a, b):
     return float(abs(D[ao_loc[a]:ao_loc[a + 1], ao_loc[b]:ao_loc[b + 1]]).max())
 
 
-@pytest.mark.parametrize('basis', ['sto-3g', 'def2-svp'])
-def test_every_dropped_quartet_is_negligible(basis):
+# the density blocks each survival screen reads: (i,j), (k,l) for J;
+# (i,k), (j,k), (i,l), (j,l) for K (create_tasks.cu:155-163, 222-225, 86-93)
+SCREEN_BLOCKS = {'jk': ('ij', 'kl', 'ik', 'jk', 'il', 'jl'), 'j': ('ij', 'kl'),
+                 'k': ('ik', 'jk', 'il', 'jl')}
+
+
+@pytest.mark.parametrize('basis,screen', [('sto-3g', 'jk'), ('def2-svp', 'jk'),
+                                          ('sto-3g', 'j'), ('sto-3g', 'k')])
+def test_every_dropped_quartet_is_negligible(basis, screen):
     '''Brute force over every shell quartet (ij) >= (kl) of the sorted basis
     of three waters 5.5 and 7.5 Angstrom apart: a quartet that is not in the
-    surviving set -- dropped by the overlap mask, by q_cond bin 0 or by the
-    predicate -- contributes at most |(ij|kl)| max|D| <= direct_scf_tol to
-    J and K (D over the six density blocks the predicate reads), up to the
-    float32 rounding of the log-domain bound; survivors are checked to be
-    a subset of what the predicate admits.'''
+    surviving set of the screen -- dropped by the overlap mask, by q_cond
+    bin 0 or by the predicate -- contributes at most |(ij|kl)| max|D| <=
+    direct_scf_tol to what the screen builds (D over the density blocks that
+    build reads: all six for J/K, D_ij and D_kl for J, the four exchange
+    blocks for K), up to the float32 rounding of the log-domain bound.'''
     from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
     mol = _spread_waters(basis)
     sm = SortedGTO(mol)
@@ -194,7 +201,7 @@ def test_every_dropped_quartet_is_negligible(basis):
     cen = sm.bas_coords()[np.searchsorted(sm.ao_loc, np.arange(sm.nao), side='right') - 1]
     r = np.linalg.norm(cen[:, None] - cen[None], axis=-1)
     D = 1e-3 * (g + g.T) / 2 * np.exp(-.3 * r)
-    keys = set(O.survivors(sm, D[None], 1, cache).tolist())
+    keys = set(O.survivors(sm, D[None], 1, cache, screen=screen).tolist())
     n = sm.nbas
     ao = sm.ao_loc
     bas, env = sm._bas, sm.env_kernel
@@ -208,15 +215,43 @@ def test_every_dropped_quartet_is_negligible(basis):
             if bij * n * n + bkl in keys:
                 continue
             ndropped += 1
-            dmax = max(_block_max(D, ao, i, j), _block_max(D, ao, k, l),
-                       _block_max(D, ao, i, k), _block_max(D, ao, j, k),
-                       _block_max(D, ao, i, l), _block_max(D, ao, j, l))
+            sh = dict(i=i, j=j, k=k, l=l)
+            dmax = max(_block_max(D, ao, sh[b[0]], sh[b[1]]) for b in SCREEN_BLOCKS[screen])
             eri = abs(O.eri_shell(bas, env, i, j, k, l)).max()
             worst = max(worst, eri * dmax)
-    print(f'{basis}: {len(keys)} survivors, {ndropped} dropped, '
+    print(f'{basis} screen {screen}: {len(keys)} survivors, {ndropped} dropped, '
           f'max dropped |(ij|kl)| max|D| = {worst:.3e}')
     assert ndropped > len(keys) // 10
     assert worst <= tol * (1 + 1e-5)
+
+
+def test_screens_partition_the_jk_set():
+    '''The J/K set is the union of the J-only and K-only sets (the six blocks
+    are the two Coulomb blocks and the four exchange blocks), each a strict
+    subset of it for a density with long-range decay; J and K of the
+    one-sided builds equal those of the J/K build within the cutoff.'''
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    mol = _spread_waters('def2-svp')
+    sm = SortedGTO(mol)
+    cache = O.build_pairs(sm, 1e-13)
+    rng = np.random.default_rng(5)
+    g = rng.standard_normal((sm.nao, sm.nao))
+    cen = sm.bas_coords()[np.searchsorted(sm.ao_loc, np.arange(sm.nao), side='right') - 1]
+    r = np.linalg.norm(cen[:, None] - cen[None], axis=-1)
+    D = (1e-2 * (g + g.T) / 2 * np.exp(-.8 * r))[None]
+    s = {sc: set(O.survivors(sm, D, 1, cache, screen=sc).tolist()) for sc in ('jk', 'j', 'k')}
+    assert s['jk'] == s['j'] | s['k']
+    assert s['j'] < s['jk'] and s['k'] < s['jk']
+    vj, vk, n_jk, _ = O.jk_sorted(sm, D, 1, cache)
+    vj1, _, n_j, _ = O.jk_sorted(sm, D, 1, cache, with_k=False, screen='j')
+    _, vk1, n_k, _ = O.jk_sorted(sm, D, 1, cache, with_j=False, screen='k')
+    assert (n_jk, n_j, n_k) == (len(s['jk']), len(s['j']), len(s['k']))
+    assert abs(vj1 - vj).max() < 1e-11 and abs(vk1 - vk).max() < 1e-11
+    # a J-only build of a non-symmetric density symmetrises it first
+    A = D + 1e-3 * rng.standard_normal(D.shape)
+    vj0 = O.jk_sorted(sm, A, 0, cache, with_k=False)[0]
+    vj2 = O.jk_sorted(sm, A, 0, cache, with_k=False, screen='j')[0]
+    assert abs(vj2 - vj0).max() < 1e-11
 
 
 def test_overlap_mask_drops_only_negligible_pairs():
