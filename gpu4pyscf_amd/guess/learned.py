@@ -63,7 +63,9 @@ def make_dataset(seeds, shape=(2, 2, 2), spacing=3.0, basis='def2-svp', log=None
 
 def train(data, epochs=50, lr=1e-3, batch=4096, seed=0, device='cuda', log=None,
           wd=1e-4):
-    '''Train a PairMLP on (X, Y, mask); returns (net, loss history).'''
+    '''Train a PairMLP on (X, Y, mask); returns (net, loss history).  The
+    training throughput is left in net.train_stats (rows/s and the GEMM
+    FLOP rate of forward + backward, 6 flops per weight per row).'''
     torch = _torch()
     X, Y, Mk = data
     net = PairMLP(device, seed=seed, dims=(X.shape[1],) + HIDDEN + (Y.shape[1],))
@@ -71,23 +73,40 @@ def train(data, epochs=50, lr=1e-3, batch=4096, seed=0, device='cuda', log=None,
     yt = torch.as_tensor(Y, device=device)
     mt = torch.as_tensor(Mk, device=device)
     n = X.shape[0]
+    # per-row mask counts on the host: the loss scale of a batch needs no
+    # device round trip
+    row_mask = Mk.sum(axis=1).astype(np.float64)
+    total_mask = float(row_mask.sum())
     gen = torch.Generator(device='cpu').manual_seed(seed)
     hist = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     for ep in range(epochs):
-        perm = torch.randperm(n, generator=gen).to(device)
-        tot = 0.
-        for b0 in range(0, n, batch):
+        perm_cpu = torch.randperm(n, generator=gen)
+        perm = perm_cpu.to(device)
+        msum = np.add.reduceat(row_mask[perm_cpu.numpy()], np.arange(0, n, batch))
+        tot = torch.zeros((), dtype=torch.float64, device=device)
+        for nb, b0 in enumerate(range(0, n, batch)):
             idx = perm[b0:b0 + batch]
             xi, yi, mi = xb[idx], yt[idx], mt[idx]
             y = net.forward(xi)
-            scale = 1. / max(float(mi.sum()), 1.)
-            loss, dy, dyb = net.loss_grad(y, yi, mi, scale)
+            m_b = max(float(msum[nb]), 1.)
+            loss, dy, dyb = net.loss_grad(y, yi, mi, 1. / m_b)
             net.backward(dy, dyb)
             net.adamw(lr=lr, wd=wd)
-            tot += float(loss) * float(mi.sum())
-        hist.append(tot / float(mt.sum()))
+            tot += loss[0].double() * m_b
+        hist.append(float(tot) / total_mask)
         if log and (ep % 10 == 0 or ep == epochs - 1):
             log(f'[train] epoch {ep}: mse {hist[-1]:.3e}')
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    weights = sum(a * b for a, b in zip(net.dims[:-1], net.dims[1:]))
+    net.train_stats = dict(seconds=dt, rows=n, epochs=epochs, rows_per_s=n * epochs / dt,
+                           gemm_tflops=6. * weights * n * epochs / dt / 1e12)
+    if log:
+        log(f"[train] {n} rows x {epochs} epochs in {dt:.1f} s: "
+            f"{net.train_stats['rows_per_s']:.3e} rows/s, "
+            f"{net.train_stats['gemm_tflops']:.2f} TFLOP/s of forward + backward GEMMs")
     return net, hist
 
 

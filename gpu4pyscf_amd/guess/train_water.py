@@ -102,7 +102,7 @@ def main():
         guess.save(args.out)
     res = dict(n_train=len(_seeds(args.train)), rows=int(data[0].shape[0]),
                t_data_s=t_data, t_train_s=t_train, final_mse=hist[-1], first_mse=hist[0],
-               epochs=args.epochs, workers=args.workers)
+               epochs=args.epochs, workers=args.workers, train_stats=net.train_stats)
     del data
     rows = []
     for s in _seeds(args.eval):

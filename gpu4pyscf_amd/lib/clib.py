@@ -77,6 +77,27 @@ class JKBuildArgs(ctypes.Structure):
     ]
 
 
+class MDBuildArgs(ctypes.Structure):
+    '''MDBuildArgs of include/jkhip.h (the J-engine's bra-ket task).'''
+    _fields_ = [
+        ('abi_version', ctypes.c_int),
+        ('lij', ctypes.c_int), ('lkl', ctypes.c_int),
+        ('npairs_ij', ctypes.c_int), ('npairs_kl', ctypes.c_int),
+        ('pdata_ij', c_void_p), ('pdata_kl', c_void_p),
+        ('q_ij', c_void_p), ('q_kl', c_void_p),
+        ('dmax_ij', c_void_p), ('dmax_kl', c_void_p),
+        ('cutoff', ctypes.c_float), ('dm_penalty', ctypes.c_float),
+        ('dh_kl', c_void_p), ('jh_ij', c_void_p),
+        ('boys', c_void_p),
+        ('boys_rows', ctypes.c_int), ('boys_m', ctypes.c_int),
+        ('boys_step', ctypes.c_double), ('boys_inv_step', ctypes.c_double),
+        ('n_quartets', c_void_p),
+        ('quartet_hash', c_void_p),
+        ('pairs_ij', c_void_p), ('pairs_kl', c_void_p),
+        ('omega', ctypes.c_double),
+    ]
+
+
 class JKTask(ctypes.Structure):
     '''JKTask of include/jkhip.h: the per-task fields of JKBuildArgs.'''
     _fields_ = [(n, ctypes.c_int) for n in ('li', 'lj', 'lk', 'll', 'iprim', 'jprim', 'kprim',
@@ -86,8 +107,8 @@ class JKTask(ctypes.Structure):
 
 
 EXPORTS = ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
-           'jk_dm_cond_noalloc', 'jk_energy_ip1',
-           'jk_int1e', 'mlp_gemm_bf16', 'mlp_silu_backward', 'mlp_colsum', 'mlp_mse_grad',
+           'jk_dm_cond_noalloc', 'jk_energy_ip1', 'jk_md_dm_to_hermite', 'jk_md_hermite_to_j',
+           'jk_md_build_j', 'jk_int1e', 'mlp_gemm_bf16', 'mlp_silu_backward', 'mlp_colsum', 'mlp_mse_grad',
            'mlp_adamw', 'mlp_to_bf16', 'jk_last_error', 'jk_abi_version')
 
 _lib = None
@@ -107,6 +128,14 @@ def load():
                  'jk_dm_cond_noalloc', 'jk_energy_ip1', 'jk_abi_version'):
         getattr(lib, name).restype = ctypes.c_int
     lib.jk_build.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p]
+    lib.jk_md_build_j.restype = ctypes.c_int
+    lib.jk_md_build_j.argtypes = [ctypes.POINTER(MDBuildArgs), c_void_p]
+    for name in ('jk_md_dm_to_hermite', 'jk_md_hermite_to_j'):
+        f = getattr(lib, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [c_void_p, c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, c_void_p,
+                      ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p, ctypes.c_int,
+                      c_void_p]
     lib.jk_energy_ip1.argtypes = [ctypes.POINTER(JKBuildArgs), c_void_p, ctypes.c_double,
                                   ctypes.c_double, c_void_p]
     lib.jk_pair_qcond.argtypes = [c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p,

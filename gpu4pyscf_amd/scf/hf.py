@@ -126,6 +126,9 @@ class RHF:
     # (gpu4pyscf/scf/hf.py:91-109), a J-only build and a K-only build, each
     # screened on its own blocks (True)
     split_jk = False
+    # with split_jk, J from the McMurchie-Davidson J-engine
+    # (gpu4pyscf/scf/hf.py:947-958) instead of the Rys J-only build
+    use_j_engine = False
 
     def __init__(self, mol):
         self.mol = mol
@@ -143,6 +146,7 @@ class RHF:
         self.history = []           # energy after the initial build and each cycle
         self.history_t = []         # seconds since kernel() started, per entry
         self.jk_hook = None
+        self._jopt = None
 
     # -- integrals -------------------------------------------------------
     @property
@@ -150,6 +154,20 @@ class RHF:
         if self._opt is None:
             self._opt = jk_mod._VHFOpt(self.mol, self.direct_scf_tol).build()
         return self._opt
+
+    @property
+    def jopt(self):
+        if self._jopt is None:
+            from gpu4pyscf_amd.scf import j_engine
+            self._jopt = j_engine._VHFOpt(self.mol, self.direct_scf_tol).build()
+        return self._jopt
+
+    def get_j(self, mol=None, dm=None, hermi=1):
+        '''J of ``dm`` by the J-engine (gpu4pyscf/scf/hf.py:947-958).'''
+        jopt = self.jopt
+        dms = jopt.apply_coeff_C_mat_CT(dm.reshape(-1, dm.shape[-1], dm.shape[-1]))
+        vj = jopt.get_j(dms, hermi)
+        return jopt.apply_coeff_CT_mat_C(vj).reshape(dm.shape)
 
     def get_hcore_ovlp(self):
         S, T, V = int1e(self.mol, self.opt)
@@ -177,7 +195,10 @@ class RHF:
         incremental = dm_last is not None and vhf_last is not None
         ddm = dm - dm_last if incremental else dm
         if self.split_jk:
-            vj = self.get_jk(self.mol, ddm, hermi=1, with_k=False, screen='j')[0]
+            if self.use_j_engine:
+                vj = self.get_j(self.mol, ddm, hermi=1)
+            else:
+                vj = self.get_jk(self.mol, ddm, hermi=1, with_k=False, screen='j')[0]
             vk = self.get_jk(self.mol, ddm, hermi=1, with_j=False, screen='k')[1]
         else:
             vj, vk = self.get_jk(self.mol, ddm, hermi=1)

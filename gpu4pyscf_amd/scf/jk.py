@@ -127,6 +127,8 @@ class _VHFOpt:
         self._streams = {}
         # copies of vj/vk the class kernels add into (one per XCD)
         self.n_repl = NREPL
+        # optional predicate on (li, lj, lk, ll): the class tasks to run
+        self.task_filter = None
 
     def profile_results(self):
         '''Per class task: (summed kernel milliseconds, surviving quartets,
@@ -157,7 +159,7 @@ class _VHFOpt:
         t0 = time.perf_counter()
         lib = clib.load()
         dev = _device()
-        mol = self.sorted_mol = SortedGTO(self.mol, diffuse_cutoff=0.3)
+        mol = self.sorted_mol = self._make_sorted_mol()
         lmax = int(mol.uniq_l_ctr[:, 0].max())
         if lmax > LMAX:
             # the reference evaluates l > 4 shells with PySCF on the CPU
@@ -218,7 +220,7 @@ class _VHFOpt:
         for (i, j), p in zip(keys, plist):
             q = q_all[pos:pos+len(p)]
             pos += len(p)
-            idx = group_by_split_points(q, split)
+            idx = self._order_pairs(q, split)
             pij = p[idx].astype(np.uint32)
             qij = q[idx].astype(np.float32)
             cache[i, j] = (pij, qij, qij)
@@ -241,6 +243,14 @@ class _VHFOpt:
         self.device = dev
         self.timings['build'] = time.perf_counter() - t0
         return self
+
+    def _make_sorted_mol(self):
+        return SortedGTO(self.mol, diffuse_cutoff=0.3)
+
+    def _order_pairs(self, q, split):
+        '''Order of a group pair's pairs: descending 4-wide q_cond bins, bin 0
+        dropped.'''
+        return group_by_split_points(q, split)
 
     # -- AO transforms (gpu4pyscf/scf/jk.py:444-460) ---------------------
     @property
@@ -267,12 +277,17 @@ class _VHFOpt:
         (gpu4pyscf/scf/jk.py:520-524); empty pair lists are skipped.'''
         mol = self.sorted_mol
         ng = len(mol.l_ctr_counts)
+        ls = mol.uniq_l_ctr[:, 0]
         out = []
         for i in range(ng):
             for j in range(i + 1):
                 for k in range(i + 1):
                     for l in range(k + 1):
-                        if len(self.bas_pair_cache[i, j][0]) and len(self.bas_pair_cache[k, l][0]):
+                        if not (len(self.bas_pair_cache[i, j][0]) and
+                                len(self.bas_pair_cache[k, l][0])):
+                            continue
+                        if self.task_filter is None or self.task_filter(
+                                tuple(int(ls[x]) for x in (i, j, k, l))):
                             out.append((i, j, k, l))
         return out
 

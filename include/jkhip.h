@@ -162,6 +162,71 @@ typedef struct {
 int jk_build_tasks(const JKBuildArgs *args, const JKTask *tasks, int ntasks,
                    void *const *streams, int nstreams);
 
+/* ---- McMurchie-Davidson J-engine (basis of single-primitive shells,
+ * SortedGTO(decontract=True)).  Replaces the reference's MD J path
+ * (gpu4pyscf/scf/j_engine.py:50-236; dm_to_Rt / MD_build_j / Rt_to_dm of
+ * gpu4pyscf/lib/gvhf-md, bound at j_engine.py:44-47). */
+
+/* Hermite densities of one group pair's screened pairs (ish >= jsh, shells
+ * of angular momenta li >= lj, li <= 4, li + lj <= 6):
+ * dh[p*NH + h] = K_p (-1)^(t+u+v) sum_{c in ish, d in jsh} D~_cd E^{cd}_{tuv}
+ * with NH = (L+1)(L+2)(L+3)/6, L = li + lj, (t, u, v) of Hermite index h in
+ * the order total ascending, t descending, u descending; K_p the pair
+ * prefactor of jk_pair_data (pdata, one primitive pair per pair) and
+ * D~_cd = D_cd + D_dc for ish != jsh.  dm: nao x nao sorted Cartesian.
+ * Replaces dm_to_Rt (gpu4pyscf/lib/gvhf-md/contract_Et.cu, called at
+ * gpu4pyscf/scf/j_engine.py:380-394). */
+int jk_md_dm_to_hermite(double *dh, const double *dm, int nao, const uint32_t *pairs,
+                        int npairs, const double *pdata, int li, int lj, const int32_t *bas,
+                        const double *env, const int32_t *ao_loc, int nbas, void *stream);
+
+/* vj[u, v] += K_p sum_h E^{uv}_h jh[p*NH + h] for u in ish, v in jsh of every
+ * pair p (and vj[v, u] for ish != jsh); plain adds (each pair owns its
+ * block).  Replaces Rt_to_dm (gpu4pyscf/scf/j_engine.py:398-430). */
+int jk_md_hermite_to_j(double *vj, const double *jh, int nao, const uint32_t *pairs,
+                       int npairs, const double *pdata, int li, int lj, const int32_t *bas,
+                       const double *env, const int32_t *ao_loc, int nbas, void *stream);
+
+/* One bra group pair against one ket group pair:
+ * jh_ij[p*NH_ij + h] += sum_{q surviving}
+ *   (a_p + a_q)^-1/2 sum_g R_{h+g}(a_p a_q / (a_p + a_q), P_p - Q_q) dh_kl[q*NH_kl + g],
+ * where (p, q) survives the J-only predicate of RYS_build_j in its float32
+ * order (create_tasks.cu:213-225): kl_cutoff = cutoff - q_ij,
+ * q_kl + dm_penalty >= kl_cutoff and max(dmax_ij, dmax_kl) > kl_cutoff - q_kl.
+ * Both pair lists sorted by descending q; lij + lkl <= MD_LMAX (4).  boys: the
+ * Boys grid of gen_md_boys.py (boys_rows x boys_m, F_m at g * boys_step).
+ * Replaces MD_build_j (gpu4pyscf/lib/gvhf-md/md_j_driver.cu, called per task
+ * at gpu4pyscf/scf/j_engine.py:199-222), which symmetrises over (ij) <-> (kl);
+ * here every bra pair sums over every ket pair. */
+typedef struct {
+    int abi_version;
+    int lij, lkl;
+    int npairs_ij, npairs_kl;
+    const double *pdata_ij, *pdata_kl;     /* device, jk_pair_data records */
+    const float *q_ij, *q_kl;              /* device, log sqrt max|(ij|ij)| */
+    const float *dmax_ij, *dmax_kl;        /* device, log max|D| of each pair block */
+    float cutoff;                          /* log(direct_scf_tol) */
+    float dm_penalty;                      /* max of the density's dm_cond */
+    const double *dh_kl;                   /* device, jk_md_dm_to_hermite output */
+    double *jh_ij;                         /* device, accumulated */
+    const double *boys;                    /* device */
+    int boys_rows, boys_m;
+    double boys_step, boys_inv_step;
+    /* survivor count and set checksum (sum mod 2^64 of the SplitMix64
+     * finaliser of bas_ij << 32 | bas_kl, as JKBuildArgs.quartet_hash) over
+     * the surviving (ij, kl) with bas_ij >= bas_kl: the canonical quartets
+     * of the J-only set; device counters or NULL.  pairs_ij / pairs_kl (the
+     * pair ids) are read only for them. */
+    unsigned long long *n_quartets;
+    unsigned long long *quartet_hash;
+    const uint32_t *pairs_ij, *pairs_kl;
+    /* operator: 1/r (0), erf(omega r)/r (omega > 0), erfc(|omega| r)/r
+     * (omega < 0), the molecule's omega as in the reference's J-engine */
+    double omega;
+} MDBuildArgs;
+
+int jk_md_build_j(const MDBuildArgs *args, void *stream);
+
 /* Primitive-pair records of a pair list, geometry-only data precomputed once
  * per molecule: out[(p*iprim*jprim + ip*jprim + jp)*6 + (0..5)] =
  * (a, 1/a, Px, Py, Pz, K) with a = ai + aj, P = (ai Ri + aj Rj)/a and

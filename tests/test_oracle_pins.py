@@ -280,3 +280,22 @@ def test_flops_model():
     # (pp|pp): n=3, Nf=81, G2=9
     assert quartet_flops((1, 1, 1, 1), (1, 1, 1, 1)) == 3 * (243 + 81) + 12 * 81
     assert quartet_digest_bytes((0, 0, 0, 0)) == 96
+
+
+def test_md_boys_grid_matches_incomplete_gamma():
+    '''The J-engine's Boys grid (csrc/gen_md_boys.py, mpmath) against
+    scipy's regularised lower incomplete gamma: F_m(x) =
+    gamma(m+1/2) P(m+1/2, x) / (2 x^(m+1/2)).'''
+    import os
+    from scipy.special import gammainc, gamma
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        'gpu4pyscf_amd', 'csrc', 'md_boys_table.npz')
+    t = np.load(path)
+    b, step = t['boys'], float(t['step'])
+    assert b.shape == (601, 20)
+    assert np.allclose(b[0], 1. / (2 * np.arange(20) + 1), rtol=0, atol=1e-15)
+    for g in (1, 7, 123, 333, 600):
+        x = g * step
+        for m in (0, 3, 12, 19):
+            ref = gamma(m + .5) * gammainc(m + .5, x) / (2 * x**(m + .5))
+            assert abs(b[g, m] - ref) <= 1e-13 * ref, (g, m)
