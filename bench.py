@@ -252,10 +252,16 @@ def main():
         f'{tflops(total_flops, dt * 1e3):.2f} TFLOP/s model over the wall time')
     log(f'[bench] one-stream build: kernels {serial_ms:.1f} ms; dominant kernel {dom_s}: '
         f'{s_ms:.1f} ms  {tflops(s_flops, s_ms):.2f} TFLOP/s  avg launch {s_ms / s_n * 1e3:.1f} us')
-    for key in sorted(by_serial, key=lambda k: -by_serial[k][0])[:20]:
+    for key in sorted(by_serial, key=lambda k: -by_serial[k][0])[:40]:
         v = by_serial[key]
         log(f'    {key}: {v[0]:8.2f} ms  {tflops(v[1], v[0]):.2f} TFLOP/s  '
             f'avg launch {v[0] / v[2] * 1e3:.1f} us  launches {v[2]}  quartets {v[3]}')
+    for kind in ('jk_lane_kernel', 'jk_team_kernel', 'jk_generic_kernel'):
+        sel = [v for k, v in by_serial.items() if k.startswith(kind)]
+        if sel:
+            ms, fl = sum(v[0] for v in sel), sum(v[1] for v in sel)
+            log(f'    all {kind}: {ms:.1f} ms, {tflops(fl, ms):.2f} TFLOP/s, '
+                f'{sum(v[3] for v in sel)} quartets')
     # the same incremental build with FP32 quartets below fp32_tol
     vj64, vk64 = opt.get_jk(dms_c, 1)
     opt.get_jk(dms_c, 1, fp32_tol=args.fp32_tol)

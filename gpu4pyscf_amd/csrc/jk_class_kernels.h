@@ -46,7 +46,8 @@
 // classes whose Cartesian block plus three 2D tables fit in LANE_MAX
 // doubles run one quartet per lane, the others on teams of lanes.
 // Measured (ms per 32-water def2-TZVP build): 120 -> 720, 150 -> 679,
-// 170 -> 660 (adds the (dd|ps)-sized classes), 240 -> 682.
+// 170 -> 660 (adds the (dd|ps)-sized classes), 240 -> 682; 64-water
+// def2-TZVP build: 95 -> 3261, 120 -> 3271, 170 -> 3134.
 #ifndef LANE_MAX
 #define LANE_MAX        170
 #endif
@@ -88,6 +89,14 @@
 #ifndef TEAM_MIN_WAVES
 #define TEAM_MIN_WAVES  1
 #endif
+// team kernel: evaluate several primitive quartets of a quartet per pass
+// (roots, 2D tables and triple products of PB primitive quartets between
+// three block barriers) instead of one.  Off: the larger LDS footprint cost
+// more than the barriers it saves (64-water def2-TZVP build, team kernels on
+// one stream: 1082 ms without, 1122 ms with)
+#ifndef TEAM_PRIM_BATCH
+#define TEAM_PRIM_BATCH 0
+#endif
 // LDS rows of the per-lane kernel: 2 = K_ik/K_jk and K_il/K_jl, 1 = K_il/K_jl
 // only; LANE_DROWS also stages the density rows (883 ms per build; 1061
 // with K_il/K_jl rows only, 887 without the density rows)
@@ -96,6 +105,15 @@
 #endif
 #ifndef LANE_DROWS
 #define LANE_DROWS      1
+#endif
+// K_ik / K_jk LDS rows of the per-lane kernel: 1 = the lanes of a run of
+// equal k shell first sum their values across the wavefront and the run's
+// last lane adds once (no same-address LDS atomics within the run), 0 = every
+// lane adds its own values.  Measured on the 64-water def2-TZVP build: 3130 ms
+// with 0, 3229 ms with 1 (LDS bank conflicts are 0.05-1.5 % of the time of
+// these kernels; the cross-lane sums cost more than the conflicts they avoid)
+#ifndef LANE_KIK_SEGSUM
+#define LANE_KIK_SEGSUM 0
 #endif
 
 void jk_set_error(const char *msg);
@@ -192,15 +210,30 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Inclusive segmented sum over lanes with equal, contiguous segment ids:
-// the last lane of a segment receives the segment total.
-__device__ __forceinline__ double seg_sum(double v, int seg, int lane)
+// Inclusive segmented sum over lanes with equal, contiguous segment ids (the
+// last lane of a segment receives the segment total), with the segment
+// comparisons done once: same[s] says whether lane - 2^s belongs to this
+// lane's segment.
+struct SegMask { bool same[6]; };
+
+__device__ __forceinline__ SegMask seg_mask(int seg, int lane)
+{
+    SegMask m;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int off = 1 << s;
+        const int o = __shfl_up(seg, off, WAVE);
+        m.same[s] = lane >= off && o == seg;
+    }
+    return m;
+}
+
+__device__ __forceinline__ double seg_sum(double v, const SegMask &m)
 {
 #pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-        double u = __shfl_up(v, off, WAVE);
-        int s = __shfl_up(seg, off, WAVE);
-        if (lane >= off && s == seg) v += u;
+    for (int s = 0; s < 6; ++s) {
+        const double u = __shfl_up(v, 1 << s, WAVE);
+        if (m.same[s]) v += u;
     }
     return v;
 }
@@ -810,6 +843,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
     const int run = __popcll(heads & (lt_mask | (1ull << lane)));
     const int run_next = __shfl_down(run, 1, WAVE);
     const bool tail = active && (lane == nb - 1 || run_next != run);
+    const SegMask runs = seg_mask(run, lane);
     // columns of this quartet's k and l AOs in the LDS density rows
     const int dk = k0 - a.k_ao0, dl = kr.loff + l0 - a.l_ao0;
     for (int id = 0; id < a.n_dm; ++id) {
@@ -847,7 +881,13 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                         kik[k] = fma(g, D[(j0+j)*nao + l0+l], kik[k]);
                     }
                 }
-                if (kr.kcols) {
+                if (kr.kcols && LANE_KIK_SEGSUM) {
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k) {
+                        double v = seg_sum(kik[k], runs);
+                        if (tail) lds_add(kr.p + i * kr.width + k0 + k - a.k_ao0, v);
+                    }
+                } else if (kr.kcols) {
                     if (active) {
 #pragma unroll
                         for (int k = 0; k < NFK; ++k)
@@ -856,7 +896,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 } else {
 #pragma unroll
                     for (int k = 0; k < NFK; ++k) {
-                        double v = seg_sum(kik[k], run, lane);
+                        double v = seg_sum(kik[k], runs);
                         if (tail) atomic_add(K + (i0+i)*nao + k0+k, v);
                     }
                 }
@@ -899,7 +939,13 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                         kjk[k] = fma(g, D[(i0+i)*nao + l0+l], kjk[k]);
                     }
                 }
-                if (kr.kcols) {
+                if (kr.kcols && LANE_KIK_SEGSUM) {
+#pragma unroll
+                    for (int k = 0; k < NFK; ++k) {
+                        double v = seg_sum(kjk[k], runs);
+                        if (tail) lds_add(kr.p + (NFI + j) * kr.width + k0 + k - a.k_ao0, v);
+                    }
+                } else if (kr.kcols) {
                     if (active) {
 #pragma unroll
                         for (int k = 0; k < NFK; ++k)
@@ -908,7 +954,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 } else {
 #pragma unroll
                     for (int k = 0; k < NFK; ++k) {
-                        double v = seg_sum(kjk[k], run, lane);
+                        double v = seg_sum(kjk[k], runs);
                         if (tail) atomic_add(K + (j0+j)*nao + k0+k, v);
                     }
                 }
@@ -1097,6 +1143,16 @@ struct TeamShape {
     static constexpr int NQ1 = (6144 / PER_Q) > 0 ? (6144 / PER_Q) : 1;   // 48 KB of LDS
     static constexpr int NQ = NQ0 < NQ1 ? NQ0 : NQ1;
     static constexpr int TS = TB / NQ;
+    // primitive quartets per pass: the 3N lanes of one primitive quartet's
+    // 2D tables are replicated PB times over the team, within the same
+    // 48 KB of LDS (one pass of roots, tables and triple products, three
+    // block barriers, per PB primitive quartets)
+    static constexpr int PB_LANES = TS / (3 * N) > 0 ? TS / (3 * N) : 1;
+    static constexpr int PB_LDS0 = 6144 / (NQ * G_DBL);
+    static constexpr int PB_LDS = PB_LDS0 > 0 ? PB_LDS0 : 1;
+    static constexpr int PB = TEAM_PRIM_BATCH ? (PB_LANES < PB_LDS ? PB_LANES : PB_LDS) : 1;
+    static constexpr int G_ALL = PB * G_DBL;
+    static constexpr int PER_QB = G_ALL > S_DBL ? G_ALL : S_DBL;
     // per-team staging of the density blocks of the current quartet:
     // D_kl, then rows r of i (r < NFI) and j with their k and l columns,
     // then D_ij (used when several densities are digested)
@@ -1125,7 +1181,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     __shared__ uint32_t queue[2 * TB];
     __shared__ int wave_cnt[NW + 1];
     __shared__ int stop_flag;
-    __shared__ double lds[NQ * S::PER_Q];
+    __shared__ double lds[NQ * S::PER_QB];
     __shared__ double dstage[NQ * S::DSZ];
     __shared__ double dij_blk[NFI * NFJ];     // D_ij of the work item (one density)
 
@@ -1144,9 +1200,10 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     ShellPair sp;
     load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    double *my = lds + (team < NQ ? team : 0) * S::PER_Q;
-    double *rw = my;                  // [2N]
-    double *gbuf = my + 2 * N;        // [N][3][GS]
+    constexpr int PB = S::PB;
+    double *my = lds + (team < NQ ? team : 0) * S::PER_QB;
+    // per primitive quartet p < PB of a pass: roots and weights at
+    // my + p * G_DBL ([2N]), then its 2D tables ([N][3][GS])
     const int row = tl;               // (i, j) row owned by this lane
     const bool has_row = team < NQ && row < NROWS;
     const int irow = has_row ? row / NFJ : 0, jrow = has_row ? row % NFJ : 0;
@@ -1217,75 +1274,91 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
             double acc[NFKL];
 #pragma unroll
             for (int n = 0; n < NFKL; ++n) acc[n] = 0.;
-            for (int kp = 0; kp < nkl; ++kp) {
+            const int npq = nij * nkl;
+            // primitive-pair quantities of primitive quartet pq (ij fastest)
+            auto prim = [&](int pq, double &aij, double &raij, double &akl, double &rakl,
+                            double *PA, double *PQ, double *QC, double &r_aa, double &theta,
+                            double &rr, double &fac) {
+                const int kp = pq / nij, ip = pq - kp * nij;
                 const double *qr = pdkl + kp * PREC;
-                const double akl = qr[0], rakl = qr[1];
-                const double Q[3] = {qr[2], qr[3], qr[4]};
-                const double Kkl = qr[5] * fac_sym;
-                for (int ip = 0; ip < nij; ++ip) {
-                    const double *pr = pdij + ip * PREC;
-                    const double aij = pr[0], raij = pr[1];
-                    double PQ[3], PA[3], QC[3];
+                const double *pr = pdij + ip * PREC;
+                akl = qr[0]; rakl = qr[1];
+                aij = pr[0]; raij = pr[1];
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        PA[d] = pr[2+d] - sp.ri[d];
-                        PQ[d] = pr[2+d] - Q[d];
-                        QC[d] = Q[d] - rk[d];
+                for (int d = 0; d < 3; ++d) {
+                    PA[d] = pr[2+d] - sp.ri[d];
+                    PQ[d] = pr[2+d] - qr[2+d];
+                    QC[d] = qr[2+d] - rk[d];
+                }
+                rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
+                r_aa = 1. / (aij + akl);
+                theta = aij * akl * r_aa;
+                fac = pr[5] * qr[5] * fac_sym * sqrt(r_aa);
+            };
+            for (int pq0 = 0; pq0 < npq; pq0 += PB) {
+                const int np = min(PB, npq - pq0);
+                __syncthreads();
+                // roots: lane (p, r) for p < np, r < N
+                if (active && tl < np * N) {
+                    const int p = tl / N, r = tl - p * N;
+                    double aij, raij, akl, rakl, PA[3], PQ[3], QC[3], r_aa, theta, rr, fac;
+                    prim(pq0 + p, aij, raij, akl, rakl, PA, PQ, QC, r_aa, theta, rr, fac);
+                    double rt, wt;
+                    if (omega2 > 0.) {
+                        // long-range attenuation (see eri_lane)
+                        const double aw = omega2 / (omega2 + theta);
+                        rys_root_one<N>(a.rys, theta * rr * aw, r, rt, wt);
+                        rt *= aw; wt *= sqrt(aw);
+                    } else {
+                        rys_root_one<N>(a.rys, theta * rr, r, rt, wt);
                     }
-                    double rr = PQ[0]*PQ[0] + PQ[1]*PQ[1] + PQ[2]*PQ[2];
-                    double r_aa = 1. / (aij + akl);
-                    double theta = aij * akl * r_aa;
-                    double fac = pr[5] * Kkl * sqrt(r_aa);
-                    __syncthreads();
-                    if (active && tl < N) {
-                        double rt, wt;
-                        if (omega2 > 0.) {
-                            // long-range attenuation (see eri_lane)
-                            const double aw = omega2 / (omega2 + theta);
-                            rys_root_one<N>(a.rys, theta * rr * aw, tl, rt, wt);
-                            rt *= aw; wt *= sqrt(aw);
-                        } else {
-                            rys_root_one<N>(a.rys, theta * rr, tl, rt, wt);
-                        }
-                        rw[2*tl] = rt; rw[2*tl+1] = wt;
-                    }
-                    __syncthreads();
-                    if (active && tl < 3 * N) {
-                        int r = tl / 3, d = tl - (tl / 3) * 3;
-                        double rt_aa = rw[2*r] * r_aa;
-                        double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
-                        double b10 = .5 * raij * (1. - rt_aij);
-                        double b01 = .5 * rakl * (1. - rt_akl);
-                        double b00 = .5 * rt_aa;
-                        double g[GS];
-                        double pa = d == 0 ? PA[0] : (d == 1 ? PA[1] : PA[2]);
-                        double pq = d == 0 ? PQ[0] : (d == 1 ? PQ[1] : PQ[2]);
-                        double qc = d == 0 ? QC[0] : (d == 1 ? QC[1] : QC[2]);
-                        double rd = d == 0 ? sp.rij[0] : (d == 1 ? sp.rij[1] : sp.rij[2]);
-                        double kd = d == 0 ? rkl[0] : (d == 1 ? rkl[1] : rkl[2]);
-                        double g00 = d == 2 ? rw[2*r+1] * fac : 1.;
-                        build_g<LI, LJ, LK, LL, double>(g, g00, pa - rt_aij * pq,
-                                                qc + rt_akl * pq,
-                                                b10, b01, b00, rd, kd);
-                        double *dst = gbuf + (r * 3 + d) * GS;
+                    double *rw = my + p * S::G_DBL;
+                    rw[2*r] = rt; rw[2*r+1] = wt;
+                }
+                __syncthreads();
+                // 2D tables: lane (p, r, d) for p < np
+                if (active && tl < np * 3 * N) {
+                    const int p = tl / (3 * N), rd = tl - p * (3 * N);
+                    const int r = rd / 3, d = rd - (rd / 3) * 3;
+                    double aij, raij, akl, rakl, PA[3], PQ[3], QC[3], r_aa, theta, rr, fac;
+                    prim(pq0 + p, aij, raij, akl, rakl, PA, PQ, QC, r_aa, theta, rr, fac);
+                    const double *rw = my + p * S::G_DBL;
+                    double rt_aa = rw[2*r] * r_aa;
+                    double rt_aij = rt_aa * akl, rt_akl = rt_aa * aij;
+                    double b10 = .5 * raij * (1. - rt_aij);
+                    double b01 = .5 * rakl * (1. - rt_akl);
+                    double b00 = .5 * rt_aa;
+                    double g[GS];
+                    double pa = d == 0 ? PA[0] : (d == 1 ? PA[1] : PA[2]);
+                    double pq = d == 0 ? PQ[0] : (d == 1 ? PQ[1] : PQ[2]);
+                    double qc = d == 0 ? QC[0] : (d == 1 ? QC[1] : QC[2]);
+                    double rd_ = d == 0 ? sp.rij[0] : (d == 1 ? sp.rij[1] : sp.rij[2]);
+                    double kd = d == 0 ? rkl[0] : (d == 1 ? rkl[1] : rkl[2]);
+                    double g00 = d == 2 ? rw[2*r+1] * fac : 1.;
+                    build_g<LI, LJ, LK, LL, double>(g, g00, pa - rt_aij * pq,
+                                            qc + rt_akl * pq,
+                                            b10, b01, b00, rd_, kd);
+                    double *dst = my + p * S::G_DBL + 2 * N + (r * 3 + d) * GS;
 #pragma unroll
-                        for (int n = 0; n < GS; ++n) dst[n] = g[n];
-                    }
-                    __syncthreads();
-                    if (active && has_row) {
-                        int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
-                        // Cartesian powers of this row's i and j (runtime lookup)
+                    for (int n = 0; n < GS; ++n) dst[n] = g[n];
+                }
+                __syncthreads();
+                if (active && has_row) {
+                    int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+                    // Cartesian powers of this row's i and j (runtime lookup)
 #pragma unroll
-                        for (int i = 0; i < NFI; ++i)
-                            if (i == irow) { ax = ci.x[i]; ay = ci.y[i]; az = ci.z[i]; }
+                    for (int i = 0; i < NFI; ++i)
+                        if (i == irow) { ax = ci.x[i]; ay = ci.y[i]; az = ci.z[i]; }
 #pragma unroll
-                        for (int j = 0; j < NFJ; ++j)
-                            if (j == jrow) { bx = cj.x[j]; by = cj.y[j]; bz = cj.z[j]; }
-                        const int ox = ax * SI + bx * SK, oy = ay * SI + by * SK, oz = az * SI + bz * SK;
+                    for (int j = 0; j < NFJ; ++j)
+                        if (j == jrow) { bx = cj.x[j]; by = cj.y[j]; bz = cj.z[j]; }
+                    const int ox = ax * SI + bx * SK, oy = ay * SI + by * SK, oz = az * SI + bz * SK;
+                    for (int p = 0; p < np; ++p) {
+                        const double *gb = my + p * S::G_DBL + 2 * N;
                         for (int r = 0; r < N; ++r) {
-                            const double *gx = gbuf + (r * 3 + 0) * GS + ox;
-                            const double *gy = gbuf + (r * 3 + 1) * GS + oy;
-                            const double *gz = gbuf + (r * 3 + 2) * GS + oz;
+                            const double *gx = gb + (r * 3 + 0) * GS + ox;
+                            const double *gy = gb + (r * 3 + 1) * GS + oy;
+                            const double *gz = gb + (r * 3 + 2) * GS + oz;
                             double sx[SK], sy[SK], sz[SK];
 #pragma unroll
                             for (int n = 0; n < SK; ++n) { sx[n] = gx[n]; sy[n] = gy[n]; sz[n] = gz[n]; }
@@ -1526,7 +1599,7 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
         // static LDS of the kernel: queue, wave counts, stop flag, team buffers
         size_t lds_static = sizeof(uint32_t) * 2 * S::TB + sizeof(int) * (S::TB / WAVE + 2) +
-                            sizeof(double) * (S::NQ * (S::PER_Q + S::DSZ) + S::NFI * S::NFJ) +
+                            sizeof(double) * (S::NQ * (S::PER_QB + S::DSZ) + S::NFI * S::NFJ) +
                             256;
         size_t lds = 0;
         const int use_lds = krows_mode<LI, LJ>(a, lds_static, TEAM_KROWS, false, lds);

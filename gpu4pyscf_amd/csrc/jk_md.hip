@@ -246,7 +246,9 @@ __device__ __forceinline__ void md_boys(const MDBuildArgs &a, double x, double *
 
 // Hermite Coulomb integrals R_tuv (t+u+v <= L) from the auxiliary values
 // r0[n] = R^(n)_000, by R^(n)_{t+1,u,v} = t R^(n+1)_{t-1,u,v} + X R^(n+1)_{t,u,v}
-// (likewise u with Y and v with Z), n from L-1 down to 0.
+// (likewise u with Y and v with Z), n from L-1 down to 0.  In place: level n
+// is written over level n+1 from the highest total order down, since an
+// entry of order k reads only entries of orders k-1 and k-2.
 template <int L>
 __device__ __forceinline__ void md_rtable(double (&R)[md_nh(L)], const double *r0, double X,
                                           double Y, double Z)
@@ -255,10 +257,8 @@ __device__ __forceinline__ void md_rtable(double (&R)[md_nh(L)], const double *r
     R[0] = r0[L];
 #pragma unroll
     for (int n = L - 1; n >= 0; --n) {
-        double S[md_nh(L)];
-        S[0] = r0[n];
 #pragma unroll
-        for (int h = 1; h < md_nh(L - n); ++h) {
+        for (int h = md_nh(L - n) - 1; h >= 1; --h) {
             const int t = ho.t[h], u = ho.u[h], v = ho.v[h];
             double s;
             if (t > 0) {
@@ -271,10 +271,9 @@ __device__ __forceinline__ void md_rtable(double (&R)[md_nh(L)], const double *r
                 s = Z * R[md_hidx(t, u, v - 1)];
                 if (v > 1) s = fma((double)(v - 1), R[md_hidx(t, u, v - 2)], s);
             }
-            S[h] = s;
+            R[h] = s;
         }
-#pragma unroll
-        for (int h = 0; h < md_nh(L - n); ++h) R[h] = S[h];
+        R[0] = r0[n];
     }
 }
 

@@ -32,10 +32,12 @@ def water_mol(seed, shape=(2, 2, 2), spacing=3.0, basis='def2-svp'):
     return M(atom=G.to_string(G.water_cluster(shape, spacing, seed=seed)), basis=basis)
 
 
-def converged_density(mol, conv_tol=1e-9):
+def converged_density(mol, conv_tol=1e-9, init_guess=None):
     from gpu4pyscf_amd.scf.hf import RHF
     mf = RHF(mol)
     mf.conv_tol = conv_tol
+    if init_guess is not None:
+        mf.init_guess = init_guess
     mf.kernel()
     if not mf.converged:
         raise RuntimeError('SCF for training data did not converge')
@@ -43,15 +45,16 @@ def converged_density(mol, conv_tol=1e-9):
 
 
 def make_dataset(seeds, shape=(2, 2, 2), spacing=3.0, basis='def2-svp', log=None,
-                 label_tol=1e-9):
+                 label_tol=1e-9, init_guess=None):
     '''Rows (X, Y, mask) from densities of seeded clusters converged to
-    label_tol.'''
+    label_tol (from init_guess, default the atomic guess: the converged
+    density does not depend on it, only the number of SCF cycles).'''
     Xs, Ys, Ms = [], [], []
     t0 = time.perf_counter()
     out_dim = F.max_block(water_mol(seeds[0], shape, spacing, basis))
     for n, s in enumerate(seeds):
         mol = water_mol(s, shape, spacing, basis)
-        dm, _ = converged_density(mol, label_tol)
+        dm, _ = converged_density(mol, label_tol, init_guess)
         rows, X = F.pairs_and_features(mol)
         Y, Mk = F.blocks_to_local(mol, dm, rows, out_dim)
         Xs.append(X); Ys.append(Y); Ms.append(Mk)

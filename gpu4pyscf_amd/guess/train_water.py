@@ -31,13 +31,20 @@ def _log(m):
     print(m, file=sys.stderr, flush=True)
 
 
+def _init_guess(args):
+    if not args.init_model:
+        return None
+    from gpu4pyscf_amd.guess.learned import LearnedGuess
+    return LearnedGuess.load(args.init_model)
+
+
 def _generate(args, shape):
     '''Training rows, in this process or split over child processes.'''
     seeds = _seeds(args.train)
     if args.workers <= 1:
         from gpu4pyscf_amd.guess import learned as L
         return L.make_dataset(seeds, shape, args.spacing, args.basis, log=_log,
-                              label_tol=args.label_tol)
+                              label_tol=args.label_tol, init_guess=_init_guess(args))
     tmp = tempfile.mkdtemp(prefix='guess_rows_')
     parts = np.array_split(np.asarray(seeds), args.workers)
     procs = []
@@ -49,6 +56,8 @@ def _generate(args, shape):
                '--train', f'{part[0]}:{part[-1] + 1}', '--shape', args.shape,
                '--spacing', str(args.spacing), '--basis', args.basis,
                '--label-tol', str(args.label_tol)]
+        if args.init_model:
+            cmd += ['--init-model', args.init_model]
         procs.append((subprocess.Popen(cmd), out))
     Xs, Ys, Ms = [], [], []
     for p, out in procs:
@@ -75,13 +84,16 @@ def main():
                     help='SCF convergence of the training densities')
     ap.add_argument('--workers', type=int, default=1)
     ap.add_argument('--out', default=None)
+    ap.add_argument('--init-model', default=None,
+                    help='weights of an earlier network whose guess starts the label SCFs '
+                         '(fewer cycles; the converged labels do not depend on it)')
     ap.add_argument('--gen-only', default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     shape = tuple(int(x) for x in args.shape.split('x'))
     if args.gen_only:
         from gpu4pyscf_amd.guess import learned as L
         X, Y, M = L.make_dataset(_seeds(args.train), shape, args.spacing, args.basis, log=_log,
-                                 label_tol=args.label_tol)
+                                 label_tol=args.label_tol, init_guess=_init_guess(args))
         np.savez(args.gen_only, X=X, Y=Y, M=M)
         return
     t0 = time.perf_counter()

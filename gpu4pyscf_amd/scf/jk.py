@@ -94,7 +94,10 @@ REPL_BYTES = 16 << 30
 def class_kernel_name(ls):
     '''Name of the HIP kernel template that evaluates angular class ls =
     (li, lj, lk, ll): the per-lane kernel when the Cartesian block and the
-    three 2D tables fit in LANE_MAX doubles, else the team kernel.'''
+    three 2D tables fit in LANE_MAX doubles, else the team kernel; classes
+    with a shell of l >= 4 run on the runtime-l kernel.'''
+    if max(ls) > 3:
+        return f'jk_generic_kernel<{ls[0]}, {ls[1]}, {ls[2]}, {ls[3]}>'
     nc = [(l + 1) * (l + 2) // 2 for l in ls]
     nf = nc[0] * nc[1] * nc[2] * nc[3]
     gs = (ls[0] + 1) * (ls[1] + 1) * (ls[2] + 1) * (ls[3] + 1)

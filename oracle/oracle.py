@@ -377,7 +377,7 @@ def get_jk(mol, dm, hermi=1, with_j=True, with_k=True, direct_scf_tol=1e-13,
         nao0 = shape[-1]
         dms = dm.reshape(-1, nao0, nao0)
         C = sorted_mol.coeff
-        dms_c = np.einsum('pi,nij,qj->npq', C, dms, C)
+        dms_c = C @ dms @ C.T
         if omega is None or rsh_factors(omega, lr_factor, sr_factor) == rsh_factors(mol_omega):
             vj, vk, nsurv, _ = jk_sorted(sorted_mol, dms_c, hermi, cache, direct_scf_tol,
                                          with_j, with_k, screen=screen)
@@ -392,8 +392,8 @@ def get_jk(mol, dm, hermi=1, with_j=True, with_k=True, direct_scf_tol=1e-13,
                                screen='k')[1]
     finally:
         set_operator(0.)
-    vj = np.einsum('pi,npq,qj->nij', C, vj, C).reshape(shape) if with_j else None
-    vk = np.einsum('pi,npq,qj->nij', C, vk, C).reshape(shape) if with_k else None
+    vj = (C.T @ vj @ C).reshape(shape) if with_j else None
+    vk = (C.T @ vk @ C).reshape(shape) if with_k else None
     return vj, vk
 
 
@@ -406,7 +406,7 @@ def ejk_ip1(mol, dm, j_factor=1., k_factor=1.):
     dm = np.asarray(dm, dtype=np.float64)
     dms = dm.reshape(-1, dm.shape[-1], dm.shape[-1])
     C = sm.coeff
-    dms_c = np.ascontiguousarray(np.einsum('pi,nij,qj->npq', C, dms, C))
+    dms_c = np.ascontiguousarray(C @ dms @ C.T)
     bas = np.ascontiguousarray(sm._bas, dtype=np.int32)
     env = np.ascontiguousarray(sm.env_kernel)
     aol = np.ascontiguousarray(sm.ao_loc, dtype=np.int32)
