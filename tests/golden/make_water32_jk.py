@@ -3,7 +3,7 @@
 dD = 1e-3 (G + G^T)/2 with G ~ N(0,1) from default_rng(1)), hermi=1, in the
 spherical AO basis: fingerprints of the whole matrices and ten full rows.
 
-Run in the build container (C oracle with OpenMP; ~15 minutes on 8 cores):
+Run in the build container (C oracle with OpenMP; ~25 minutes on 8 cores):
     python tests/golden/make_water32_jk.py
 '''
 import os
