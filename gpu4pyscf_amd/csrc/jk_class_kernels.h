@@ -563,7 +563,15 @@ __device__ __forceinline__ bool survives_dmax(float q_kl, float kl_cutoff, float
 }
 
 // Work item of a launch: pair_ij index and the kl range of its chunk.
-struct WorkItem { int pij, kl0, kl1; };
+// One work item: ij pair pij against kl pairs [kl0, kl1), whose k and l AOs
+// lie in [k_ao0, k_ao0 + k_nao) and [l_ao0, l_ao0 + l_nao) (the K-row
+// columns of the item).
+struct WorkItem { int pij, kl0, kl1, k_ao0, k_nao, l_ao0, l_nao; };
+
+// use_lds bits of the per-lane and team kernels
+#define KR_MODE_MASK    3       // 0 none, 1 K_il/K_jl rows, 2 all four K row blocks
+#define KR_DROWS        4       // density rows staged too
+#define KR_WIN          8       // work items = ij pairs x kl column windows
 
 // Workgroups are dispatched round-robin over the XCDs (each with its own
 // L2): block b runs on XCD b % NXCD.  With XCD_SWIZZLE the blocks of one XCD
@@ -594,12 +602,37 @@ __device__ __forceinline__ WorkItem work_item(int npairs_kl, int kl_chunk)
     int ch = idx - w.pij * nchunks;
     w.kl0 = ch * kl_chunk;
     w.kl1 = min(w.kl0 + kl_chunk, npairs_kl);
+    w.k_ao0 = w.k_nao = w.l_ao0 = w.l_nao = 0;
+    return w;
+}
+
+// work items over the kl column windows (window major, ij pair minor); the
+// K-row columns are the window's
+__device__ __forceinline__ WorkItem work_item_win(const JKBuildArgs &a)
+{
+    WorkItem w;
+    const int idx = work_index();
+    const int win = idx / a.npairs_ij;
+    w.pij = idx - win * a.npairs_ij;
+    const int32_t *d = a.kl_win + 6 * win;
+    w.kl0 = d[0]; w.kl1 = d[1];
+    w.k_ao0 = d[2]; w.k_nao = d[3]; w.l_ao0 = d[4]; w.l_nao = d[5];
+    return w;
+}
+
+// work item of a per-lane or team launch: kl chunks over the whole groups,
+// or the kl column windows
+__device__ __forceinline__ WorkItem work_item(const JKBuildArgs &a, int kl_chunk, int use_lds)
+{
+    if (use_lds & KR_WIN) return work_item_win(a);
+    WorkItem w = work_item(a.npairs_kl, kl_chunk);
+    w.k_ao0 = a.k_ao0; w.k_nao = a.k_nao; w.l_ao0 = a.l_ao0; w.l_nao = a.l_nao;
     return w;
 }
 
 // K rows of one work item accumulated in LDS (one density): rows i (NFI)
-// then j (NFJ); columns: the k group's AOs (when kcols) followed by the l
-// group's AOs.  Flushed to K with one atomic per non-zero element.
+// then j (NFJ); columns: the work item's k AOs (when kcols) followed by its
+// l AOs (the whole k and l groups, or one kl column window).  Flushed to K with one atomic per non-zero element.
 // The same row layout also holds, after the K rows, a copy of the density
 // rows i and j over the k and l group columns (dstaged), from which the
 // per-lane digestion reads D_ik, D_il, D_jk and D_jl.
@@ -609,7 +642,8 @@ struct KRows {
     double *p;          // K rows
     const double *d;    // density rows
     int width;          // doubles per row
-    int loff;           // column of the l group's first AO
+    int loff;           // column of the first l AO
+    int k_ao0, l_ao0;   // AOs of columns 0 and loff
     bool on;            // K_il / K_jl (and with kcols K_ik / K_jk) accumulated here
     bool kcols;         // K_ik / K_jk accumulated here too
     bool dstaged;       // density rows present
@@ -618,14 +652,16 @@ struct KRows {
 // use_lds: 0 none, 1 K_il/K_jl rows, 2 K_ik/K_jk and K_il/K_jl rows,
 // 2|4 the same plus the density rows
 __device__ __forceinline__ KRows make_krows(double *buf, int use_lds, int nrows,
-                                            const JKBuildArgs &a)
+                                            const WorkItem &w)
 {
     KRows kr;
-    kr.on = (use_lds & 3) != 0;
-    kr.kcols = (use_lds & 3) == 2;
-    kr.dstaged = kr.kcols && (use_lds & 4);
-    kr.loff = kr.kcols ? a.k_nao : 0;
-    kr.width = kr.loff + a.l_nao;
+    kr.on = (use_lds & KR_MODE_MASK) != 0;
+    kr.kcols = (use_lds & KR_MODE_MASK) == 2;
+    kr.dstaged = kr.kcols && (use_lds & KR_DROWS);
+    kr.loff = kr.kcols ? w.k_nao : 0;
+    kr.width = kr.loff + w.l_nao;
+    kr.k_ao0 = w.k_ao0;
+    kr.l_ao0 = w.l_ao0;
     kr.p = buf;
     kr.d = buf + nrows * kr.width;
     return kr;
@@ -641,7 +677,7 @@ __device__ __forceinline__ void krows_load_d(const KRows &kr, int nfi, int nfj, 
     for (int n = tid; n < n_all; n += nthreads) {
         int r = n / kr.width, c = n - r * kr.width;
         int row = r < nfi ? i0 + r : j0 + r - nfi;
-        int col = c < kr.loff ? a.k_ao0 + c : a.l_ao0 + c - kr.loff;
+        int col = c < kr.loff ? kr.k_ao0 + c : kr.l_ao0 + c - kr.loff;
         d[n] = a.dm[(size_t)row * a.nao + col];
     }
 }
@@ -664,7 +700,7 @@ __device__ __forceinline__ void krows_flush(const KRows &kr, int nfi, int nfj, i
         if (v != 0.) {
             int r = n / kr.width, c = n - r * kr.width;
             int row = r < nfi ? i0 + r : j0 + r - nfi;
-            int col = c < kr.loff ? a.k_ao0 + c : a.l_ao0 + c - kr.loff;
+            int col = c < kr.loff ? kr.k_ao0 + c : kr.l_ao0 + c - kr.loff;
             atomic_add(a.vk + (size_t)row * a.nao + col, v);
         }
     }
@@ -845,7 +881,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
     const bool tail = active && (lane == nb - 1 || run_next != run);
     const SegMask runs = seg_mask(run, lane);
     // columns of this quartet's k and l AOs in the LDS density rows
-    const int dk = k0 - a.k_ao0, dl = kr.loff + l0 - a.l_ao0;
+    const int dk = k0 - kr.k_ao0, dl = kr.loff + l0 - kr.l_ao0;
     for (int id = 0; id < a.n_dm; ++id) {
         const double *D = a.dm + id * nao2;
         if (a.vk) {
@@ -885,13 +921,13 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
 #pragma unroll
                     for (int k = 0; k < NFK; ++k) {
                         double v = seg_sum(kik[k], runs);
-                        if (tail) lds_add(kr.p + i * kr.width + k0 + k - a.k_ao0, v);
+                        if (tail) lds_add(kr.p + i * kr.width + k0 + k - kr.k_ao0, v);
                     }
                 } else if (kr.kcols) {
                     if (active) {
 #pragma unroll
                         for (int k = 0; k < NFK; ++k)
-                            lds_add(kr.p + i * kr.width + k0 + k - a.k_ao0, kik[k]);
+                            lds_add(kr.p + i * kr.width + k0 + k - kr.k_ao0, kik[k]);
                     }
                 } else {
 #pragma unroll
@@ -903,7 +939,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 if (active) {
 #pragma unroll
                     for (int l = 0; l < NFL; ++l) {
-                        if (kr.on) lds_add(kr.p + i * kr.width + kr.loff + l0 + l - a.l_ao0, kil[l]);
+                        if (kr.on) lds_add(kr.p + i * kr.width + kr.loff + l0 + l - kr.l_ao0, kil[l]);
                         else atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
                     }
                 }
@@ -943,13 +979,13 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
 #pragma unroll
                     for (int k = 0; k < NFK; ++k) {
                         double v = seg_sum(kjk[k], runs);
-                        if (tail) lds_add(kr.p + (NFI + j) * kr.width + k0 + k - a.k_ao0, v);
+                        if (tail) lds_add(kr.p + (NFI + j) * kr.width + k0 + k - kr.k_ao0, v);
                     }
                 } else if (kr.kcols) {
                     if (active) {
 #pragma unroll
                         for (int k = 0; k < NFK; ++k)
-                            lds_add(kr.p + (NFI + j) * kr.width + k0 + k - a.k_ao0, kjk[k]);
+                            lds_add(kr.p + (NFI + j) * kr.width + k0 + k - kr.k_ao0, kjk[k]);
                     }
                 } else {
 #pragma unroll
@@ -961,7 +997,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 if (active) {
 #pragma unroll
                     for (int l = 0; l < NFL; ++l) {
-                        if (kr.on) lds_add(kr.p + (NFI + j) * kr.width + kr.loff + l0 + l - a.l_ao0, kjl[l]);
+                        if (kr.on) lds_add(kr.p + (NFI + j) * kr.width + kr.loff + l0 + l - kr.l_ao0, kjl[l]);
                         else atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
                     }
                 }
@@ -1042,7 +1078,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     uint32_t *queue32 = queue_all[wid][1];
     const int nbas = a.nbas, nao = a.nao;
     select_replica(a);
-    WorkItem wi = work_item(a.npairs_kl, kl_chunk);
+    WorkItem wi = work_item(a, kl_chunk, use_lds);
     const uint32_t bas_ij = a.pairs_ij[wi.pij];
     const float q_ij = a.q_ij[wi.pij];
     const float kl_cutoff = a.cutoff - q_ij;
@@ -1051,7 +1087,7 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     load_shell_pair(sp, bas_ij, nbas, a.bas, a.env, a.ao_loc);
     // K rows of this work item accumulated in LDS (one density)
     extern __shared__ double kacc_dyn[];
-    const KRows kr = make_krows(kacc_dyn, use_lds, NFI + NFJ, a);
+    const KRows kr = make_krows(kacc_dyn, use_lds, NFI + NFJ, wi);
     if (kr.on) {
         krows_zero(kr, NFI + NFJ, threadIdx.x, LANE_WAVES * WAVE);
         krows_load_d(kr, NFI, NFJ, sp.i0, sp.j0, a, threadIdx.x, LANE_WAVES * WAVE);
@@ -1193,7 +1229,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     const int nbas = a.nbas, nao = a.nao;
     const size_t nao2 = (size_t)nao * nao;
     select_replica(a);
-    WorkItem wi = work_item(a.npairs_kl, kl_chunk);
+    WorkItem wi = work_item(a, kl_chunk, use_lds);
     const uint32_t bas_ij = a.pairs_ij[wi.pij];
     const float kl_cutoff = a.cutoff - a.q_ij[wi.pij];
     if (a.q_kl[wi.kl0] + a.dm_penalty + Q_COND_MARGIN < kl_cutoff) return;
@@ -1211,7 +1247,7 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
     double vj_row = 0.;
     // K rows of this work item accumulated in LDS (one density)
     extern __shared__ double kacc_dyn[];
-    const KRows kr = make_krows(kacc_dyn, use_lds, NFI + NFJ, a);
+    const KRows kr = make_krows(kacc_dyn, use_lds, NFI + NFJ, wi);
     krows_zero(kr, NFI + NFJ, tid, TB);
     for (int n = tid; n < NFI * NFJ; n += TB)
         dij_blk[n] = a.dm[(size_t)(sp.i0 + n / NFJ) * nao + sp.j0 + n % NFJ];
@@ -1458,10 +1494,10 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                                 }
                             }
                             if (m < NFK) {
-                                if (kr.kcols) lds_add(kr.p + i * kr.width + k0 + m - a.k_ao0, s);
+                                if (kr.kcols) lds_add(kr.p + i * kr.width + k0 + m - kr.k_ao0, s);
                                 else atomic_add(K + (i0+i)*nao + k0+m, s);
                             } else if (kr.on) {
-                                lds_add(kr.p + i * kr.width + kr.loff + l0 + m - NFK - a.l_ao0, s);
+                                lds_add(kr.p + i * kr.width + kr.loff + l0 + m - NFK - kr.l_ao0, s);
                             } else {
                                 atomic_add(K + (i0+i)*nao + l0+m-NFK, s);
                             }
@@ -1486,10 +1522,10 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                                 }
                             }
                             if (m < NFK) {
-                                if (kr.kcols) lds_add(kr.p + (NFI + j) * kr.width + k0 + m - a.k_ao0, s);
+                                if (kr.kcols) lds_add(kr.p + (NFI + j) * kr.width + k0 + m - kr.k_ao0, s);
                                 else atomic_add(K + (j0+j)*nao + k0+m, s);
                             } else if (kr.on) {
-                                lds_add(kr.p + (NFI + j) * kr.width + kr.loff + l0 + m - NFK - a.l_ao0, s);
+                                lds_add(kr.p + (NFI + j) * kr.width + kr.loff + l0 + m - NFK - kr.l_ao0, s);
                             } else {
                                 atomic_add(K + (j0+j)*nao + l0+m-NFK, s);
                             }
@@ -1547,21 +1583,27 @@ static size_t lds_limit()
 // the dynamic LDS size.  max_mode 1 suits kernels whose many short work
 // items would spend more on zeroing and flushing the wider rows (and on the
 // occupancy they cost) than the K_ik/K_jk atomics they save.
+// k_nao / l_nao: the K-row columns of a work item (the whole k and l groups
+// of the task, or with kl column windows the widest window, l_nao = 0 and
+// k_nao = kl_win_width).
 template <int LI, int LJ>
-static int krows_mode(const JKBuildArgs &a, size_t lds_static, int max_mode, bool drows,
-                      size_t &bytes)
+static int krows_mode(const JKBuildArgs &a, int k_nao, int l_nao, size_t lds_static,
+                      int max_mode, bool drows, size_t &bytes)
 {
     constexpr size_t NFIJ = ncart(LI) + ncart(LJ);
     bytes = 0;
-    if (!a.vk || a.n_dm != 1 || a.l_nao <= 0 || a.k_nao <= 0) return 0;
-    const size_t both = NFIJ * (size_t)(a.k_nao + a.l_nao) * sizeof(double);
-    const size_t lonly = NFIJ * (size_t)a.l_nao * sizeof(double);
+    if (!a.vk || a.n_dm != 1 || l_nao < 0 || k_nao < 0 || k_nao + l_nao <= 0) return 0;
+    const size_t both = NFIJ * (size_t)(k_nao + l_nao) * sizeof(double);
+    const size_t lonly = NFIJ * (size_t)l_nao * sizeof(double);
     if (max_mode >= 2 && drows && 2 * both + lds_static <= lds_limit()) {
         bytes = 2 * both;
-        return 2 | 4;
+        return 2 | KR_DROWS;
     }
     if (max_mode >= 2 && both + lds_static <= lds_limit()) { bytes = both; return 2; }
-    if (max_mode >= 1 && lonly + lds_static <= lds_limit()) { bytes = lonly; return 1; }
+    if (max_mode >= 1 && l_nao > 0 && lonly + lds_static <= lds_limit()) {
+        bytes = lonly;
+        return 1;
+    }
     return 0;
 }
 
@@ -1588,7 +1630,25 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
         size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * 2 * WAVE;
         size_t lds = 0;
-        const int use_lds = krows_mode<LI, LJ>(a, lds_static, LANE_KROWS, LANE_DROWS, lds);
+        const int use_lds = krows_mode<LI, LJ>(a, a.k_nao, a.l_nao, lds_static, LANE_KROWS,
+                                               LANE_DROWS, lds);
+        if (a.kl_win && LANE_KROWS >= 2 &&
+            (a.kl_win_policy == 1 || (use_lds & KR_MODE_MASK) != 2)) {
+            // K rows over one kl column window per work item
+            size_t lds_w = 0;
+            const int mode_w = krows_mode<LI, LJ>(a, a.kl_win_width, 0, lds_static, LANE_KROWS,
+                                                  LANE_DROWS, lds_w);
+            const long nblocks_w = (long)a.npairs_ij * a.n_kl_win;
+            if ((mode_w & KR_MODE_MASK) == 2 && nblocks_w <= 0x7fffffffL) {
+                JKBuildArgs aw = a;
+                aw.pairs_kl = a.pairs_kl_w;
+                aw.q_kl = a.q_kl_w;
+                aw.pdata_kl = a.pdata_kl_w;
+                hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks_w),
+                                   dim3(LANE_WAVES * WAVE), lds_w, stream, aw, 0, mode_w | KR_WIN);
+                return hipGetLastError();
+            }
+        }
         hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(LANE_WAVES * WAVE),
                            lds, stream, a, kl_chunk, use_lds);
     } else {
@@ -1602,7 +1662,8 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
                             sizeof(double) * (S::NQ * (S::PER_QB + S::DSZ) + S::NFI * S::NFJ) +
                             256;
         size_t lds = 0;
-        const int use_lds = krows_mode<LI, LJ>(a, lds_static, TEAM_KROWS, false, lds);
+        const int use_lds = krows_mode<LI, LJ>(a, a.k_nao, a.l_nao, lds_static, TEAM_KROWS,
+                                               false, lds);
         hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(S::TB),
                            lds, stream, a, kl_chunk, use_lds);
     }

@@ -28,6 +28,7 @@ from gpu4pyscf_amd.lib import clib
 __all__ = ['get_jk', 'get_j', 'get_k', '_VHFOpt']
 
 TILE = 12                 # pair enumeration tile, gpu4pyscf/scf/jk.py:51
+KL_WINDOW_AO = 384        # widest kl column window (AOs of k, and of l)
 Q_COND_MARGIN = 4.        # bin width of the descending q_cond ordering
 LMAX = clib.JK_LMAX
 
@@ -105,6 +106,55 @@ def class_kernel_name(ls):
     return f'{kind}<{ls[0]}, {ls[1]}, {ls[2]}, {ls[3]}>'
 
 
+def _set_kl_win(a, win):
+    '''kl column window fields of a JKTask / JKBuildArgs (win: an entry of
+    _VHFOpt._kl_windows or None).'''
+    if win is None:
+        a.pairs_kl_w = a.q_kl_w = a.pdata_kl_w = a.kl_win = None
+        a.n_kl_win = a.kl_win_width = 0
+    else:
+        a.pairs_kl_w, a.q_kl_w, a.pdata_kl_w, a.kl_win = (x.data_ptr() for x in win[:4])
+        a.n_kl_win, a.kl_win_width = len(win[3]), win[4]
+
+
+def kl_window_table(pairs, offs, ao_loc, nbas, k, l, W):
+    '''kl column windows of the pair list ``pairs`` (ids ksh*nbas + lsh) of
+    group pair (k, l), or None when neither group exceeds W AOs.  Each group
+    is cut into windows of whole shells, as equal as they can be, of at most
+    W AOs or one shell (the shells of a group have one size); returns (order, table, width): the stable
+    reordering of the pairs window by window (each window keeps the list's
+    descending q bins), the int32 (n_windows, 6) rows {first, end, k_ao0,
+    k_nao, l_ao0, l_nao} of the non-empty windows (JKBuildArgs.kl_win) and
+    the largest k_nao + l_nao.'''
+    aol = np.asarray(ao_loc, dtype=np.int64)
+    k_nao = int(aol[offs[k+1]] - aol[offs[k]])
+    l_nao = int(aol[offs[l+1]] - aol[offs[l]])
+    if len(pairs) == 0 or (k_nao <= W and l_nao <= W):
+        return None
+
+    def cut(g, nao):
+        nsh = int(offs[g+1] - offs[g])
+        spw = max(1, W // (nao // nsh))     # most shells per window
+        spw = -(-nsh // -(-nsh // spw))     # the same number of windows, evened out
+        return spw, -(-nsh // spw)
+    spk, _ = cut(k, k_nao)
+    spl, nwl = cut(l, l_nao)
+    p = np.asarray(pairs).astype(np.int64)
+    ksh, lsh = p // nbas, p % nbas
+    key = ((ksh - offs[k]) // spk) * nwl + (lsh - offs[l]) // spl
+    order = np.argsort(key, kind='stable')
+    ids, first = np.unique(key[order], return_index=True)
+    end = np.append(first[1:], len(p))
+    a, b = np.divmod(ids, nwl)
+    k0 = offs[k] + a * spk
+    k1 = np.minimum(k0 + spk, offs[k+1])
+    l0 = offs[l] + b * spl
+    l1 = np.minimum(l0 + spl, offs[l+1])
+    tab = np.stack([first, end, aol[k0], aol[k1] - aol[k0], aol[l0], aol[l1] - aol[l0]],
+                   axis=1).astype(np.int32)
+    return order, tab, int((tab[:, 3] + tab[:, 5]).max())
+
+
 class _VHFOpt:
     '''Screening object of the direct J/K build (gpu4pyscf/scf/jk.py:407-486).'''
 
@@ -132,6 +182,12 @@ class _VHFOpt:
         self.n_repl = NREPL
         # optional predicate on (li, lj, lk, ll): the class tasks to run
         self.task_filter = None
+        # kl column windows: group pairs wider than kl_window_ao AOs in k or
+        # l are split into windows of at most that many AOs each, so that
+        # the per-lane kernels keep a work item's K rows in LDS (used where
+        # the rows over the whole groups do not fit; kl_win_policy 1: always)
+        self.kl_window_ao = KL_WINDOW_AO
+        self.kl_win_policy = 0
 
     def profile_results(self):
         '''Per class task: (summed kernel milliseconds, surviving quartets,
@@ -238,6 +294,7 @@ class _VHFOpt:
                        'jk_pair_data')
             dev_pd[i, j] = pd
         self.bas_pair_cache = cache
+        d['kl_win'] = self._kl_windows(cache, dev_pairs, dev_q, dev_pd)
         d['pairs'] = dev_pairs
         d['q'] = dev_q
         d['pdata'] = dev_pd
@@ -249,6 +306,27 @@ class _VHFOpt:
 
     def _make_sorted_mol(self):
         return SortedGTO(self.mol, diffuse_cutoff=0.3)
+
+    def _kl_windows(self, cache, dev_pairs, dev_q, dev_pd):
+        '''kl column windows of every group pair wider than kl_window_ao
+        AOs in k or in l: {(k, l): (pairs, q, pdata, table, width)}, the
+        device copies of kl_window_table's reordering and window rows.'''
+        torch = _torch()
+        mol = self.sorted_mol
+        out = {}
+        if int(self.kl_window_ao) <= 0:
+            return out
+        for (k, l), (p, _, _) in cache.items():
+            w = kl_window_table(p, mol.group_offsets, mol.ao_loc, mol.nbas, k, l,
+                                int(self.kl_window_ao))
+            if w is None:
+                continue
+            order, tab, width = w
+            dev = dev_pairs[k, l].device
+            o = torch.as_tensor(order, device=dev)
+            out[k, l] = (dev_pairs[k, l][o].contiguous(), dev_q[k, l][o].contiguous(),
+                         dev_pd[k, l][o].contiguous(), torch.as_tensor(tab, device=dev), width)
+        return out
 
     def _order_pairs(self, q, split):
         '''Order of a group pair's pairs: descending 4-wide q_cond bins, bin 0
@@ -318,6 +396,7 @@ class _VHFOpt:
             a.q_kl = d['q'][k, l].data_ptr()
             a.pdata_ij = d['pdata'][i, j].data_ptr()
             a.pdata_kl = d['pdata'][k, l].data_ptr()
+            _set_kl_win(a, d['kl_win'].get((k, l)))
         return arr
 
     def operator_passes(self, omega=None, lr_factor=None, sr_factor=None):
@@ -436,6 +515,7 @@ class _VHFOpt:
         args.n_repl = n_repl
         args.repl_stride = dms.numel()
         args.screen = clib.JK_SCREEN[screen]
+        args.kl_win_policy = int(self.kl_win_policy)
         uniq = mol.uniq_l_ctr
         goff = mol.group_offsets
         g_ao0 = mol.ao_loc[goff[:-1]]
@@ -503,6 +583,7 @@ class _VHFOpt:
                     args.q_kl = qkl.data_ptr()
                     args.pdata_ij = d['pdata'][i, j].data_ptr()
                     args.pdata_kl = d['pdata'][k, l].data_ptr()
+                    _set_kl_win(args, d['kl_win'].get((k, l)))
                     if first and (time_classes or prof):
                         e0 = torch.cuda.Event(enable_timing=True)
                         e1 = torch.cuda.Event(enable_timing=True)

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  10
+#define JK_ABI_VERSION  11
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         5       /* max angular momentum on the GPU path */
 
@@ -116,6 +116,25 @@ typedef struct {
      * (RYS_build_j, create_tasks.cu:222-225); JK_SCREEN_K: D_ik, D_jk, D_il
      * or D_jl, vj must be NULL (RYS_build_k, create_tasks.cu:86-93). */
     int screen;
+    /* Column windows of the kl list, for K rows of a work item accumulated
+     * in LDS when the rows over the whole k and l groups exceed the LDS of a
+     * workgroup (NULL / 0: no windows).  pairs_kl_w, q_kl_w and pdata_kl_w
+     * hold the npairs_kl kl pairs ordered window by window, each window's
+     * pairs in the descending q order of pairs_kl; window w is
+     * kl_win[6w .. 6w+5] = {first pair, end pair, k_ao0, k_nao, l_ao0,
+     * l_nao}: its pairs have k in AOs [k_ao0, k_ao0+k_nao) and l in
+     * [l_ao0, l_ao0+l_nao); kl_win_width = max over windows of k_nao + l_nao.
+     * kl_win_policy 0: the per-lane kernels use the windows only when the K
+     * rows over the whole groups do not fit; 1: whenever the window rows fit
+     * (tests).  No counterpart in the reference, whose K rows are
+     * accumulated in registers and global atomics. */
+    const uint32_t *pairs_kl_w;
+    const float *q_kl_w;
+    const double *pdata_kl_w;
+    const int32_t *kl_win;          /* device */
+    int n_kl_win;
+    int kl_win_width;
+    int kl_win_policy;
 } JKBuildArgs;
 
 #define JK_SCREEN_JK    0
@@ -149,6 +168,13 @@ typedef struct {
     const uint32_t *pairs_ij, *pairs_kl;
     const float *q_ij, *q_kl;
     const double *pdata_ij, *pdata_kl;
+    /* kl column windows, as in JKBuildArgs (NULL / 0: none) */
+    const uint32_t *pairs_kl_w;
+    const float *q_kl_w;
+    const double *pdata_kl_w;
+    const int32_t *kl_win;
+    int n_kl_win;
+    int kl_win_width;
 } JKTask;
 
 /* All class tasks of one build: the task loop of gpu4pyscf/scf/jk.py:554-585

@@ -92,7 +92,7 @@ def test_jk_build_tasks_validates_on_host():
     task jk_build rejects, before any launch.'''
     from gpu4pyscf_amd.lib import clib
     lib = clib.load()
-    assert ctypes.sizeof(clib.JKTask) == 14 * 4 + 6 * 8
+    assert ctypes.sizeof(clib.JKTask) == 14 * 4 + 6 * 8 + 4 * 8 + 2 * 4
     a = clib.JKBuildArgs()
     a.abi_version = clib.JK_ABI_VERSION
     tasks = (clib.JKTask * 2)()
@@ -103,6 +103,46 @@ def test_jk_build_tasks_validates_on_host():
     assert lib.jk_build_tasks(ctypes.byref(a), tasks, 2, streams, 1) == 2
     assert b'angular class' in lib.jk_last_error()
     assert lib.jk_build_tasks(ctypes.byref(a), tasks, 0, streams, 1) == 0
+
+
+@pytest.mark.parametrize('W', [6, 10, 16])
+def test_kl_window_table_partitions_the_pairs(W):
+    '''The kl column windows: a stable reordering whose windows are
+    contiguous, non-empty, within W AOs per group, and hold every pair's k
+    and l AOs inside their columns.'''
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    from gpu4pyscf_amd.scf import jk
+    from oracle import oracle as O
+    from tests.molecules import ATOM6
+    mol = SortedGTO(M(atom=ATOM6, basis='def2-tzvp', unit='B'), diffuse_cutoff=0.3)
+    cache = O.build_pairs(mol)
+    aol = np.asarray(mol.ao_loc)
+    nbas = mol.nbas
+    n_windowed = 0
+    for (k, l), (p, q) in cache.items():
+        w = jk.kl_window_table(p, mol.group_offsets, aol, nbas, k, l, W)
+        if w is None:
+            assert len(p) == 0 or (aol[mol.group_offsets[k+1]] - aol[mol.group_offsets[k]] <= W and
+                                   aol[mol.group_offsets[l+1]] - aol[mol.group_offsets[l]] <= W)
+            continue
+        n_windowed += 1
+        order, tab, width = w
+        assert sorted(order.tolist()) == list(range(len(p)))
+        assert tab[0, 0] == 0 and tab[-1, 1] == len(p)
+        assert np.all(tab[1:, 0] == tab[:-1, 1]) and np.all(tab[:, 1] > tab[:, 0])
+        nk = (aol[mol.group_offsets[k]+1] - aol[mol.group_offsets[k]])
+        nl = (aol[mol.group_offsets[l]+1] - aol[mol.group_offsets[l]])
+        assert np.all(tab[:, 3] <= max(W, nk)) and np.all(tab[:, 5] <= max(W, nl))
+        assert width == int((tab[:, 3] + tab[:, 5]).max())
+        po = p[order].astype(np.int64)
+        for f, e, k0, kn, l0, ln in tab:
+            ks, ls = po[f:e] // nbas, po[f:e] % nbas
+            assert np.all(aol[ks] >= k0) and np.all(aol[ks + 1] <= k0 + kn)
+            assert np.all(aol[ls] >= l0) and np.all(aol[ls + 1] <= l0 + ln)
+            # stable: the window keeps the list's order (descending q bins)
+            assert np.all(np.diff(order[f:e]) > 0)
+    assert n_windowed > 0
 
 
 def test_tile_pairs_and_bins_match_oracle():

@@ -230,3 +230,38 @@ def test_one_sided_builds_match_oracle_sets_and_values(screen, hermi):
         assert abs(jk.get_j(mol, dm, hermi=hermi, vhfopt=opt) - rj).max() < 1e-10
     else:
         assert abs(jk.get_k(mol, dm, hermi=hermi, vhfopt=opt) - rk).max() < 1e-10
+
+
+@pytest.mark.parametrize('W,with_j', [(12, True), (40, True), (12, False)])
+def test_kl_column_windows_match_oracle(W, with_j):
+    '''kl column windows forced on every per-lane launch that can hold the
+    window's K rows in LDS (kl_win_policy 1, windows of at most W AOs): the
+    surviving quartets of every class task (count and set checksum) equal
+    the oracle's, and J/K equal the oracle's to 1e-10, on the 8-water
+    def2-SVP cluster (192 AOs; its widest groups have 64 AOs).'''
+    import torch
+    from gpu4pyscf_amd.scf import jk
+    mol, dm = _water8_svp()
+    opt = jk._VHFOpt(mol)
+    opt.kl_window_ao = W
+    opt.kl_win_policy = 1
+    opt.build()
+    assert len(opt._dev['kl_win']) > 0
+    nwin = {k: len(v[3]) for k, v in opt._dev['kl_win'].items()}
+    print(f'[kl windows W={W} with_j={with_j}] windows per group pair {nwin}', flush=True)
+    sm = opt.sorted_mol
+    dms = opt.apply_coeff_C_mat_CT(torch.as_tensor(dm, device='cuda')[None])
+    screen = None if with_j else 'k'
+    vj, vk = opt.get_jk(dms, 1, with_j=with_j, screen=screen, count_quartets=True,
+                        hash_quartets=True)
+    got = {t: (c, opt.last_task_hashes[t]) for t, c in opt.last_task_counts.items() if c}
+    cache = {k: (v[0], v[1]) for k, v in opt.bas_pair_cache.items()}
+    ref = O.quartet_sets(sm, dms.cpu().numpy(), 1, cache, opt.direct_scf_tol,
+                         screen=screen or 'jk')
+    assert got == ref
+    rj, rk = O.get_jk(mol, dm, hermi=1, with_j=with_j, screen=screen or 'jk')
+    vk = opt.apply_coeff_CT_mat_C(vk)[0].cpu().numpy()
+    assert abs(vk - rk).max() < 1e-10
+    if with_j:
+        vj = opt.apply_coeff_CT_mat_C(vj)[0].cpu().numpy()
+        assert abs(vj - rj).max() < 1e-10
