@@ -106,6 +106,11 @@
 #ifndef LANE_DROWS
 #define LANE_DROWS      1
 #endif
+// rank the per-lane LDS row candidates by the workgroups per CU their LDS
+// allows (0: take the largest rows that fit)
+#ifndef LANE_KROWS_OCC
+#define LANE_KROWS_OCC  1
+#endif
 // K_ik / K_jk LDS rows of the per-lane kernel: 1 = the lanes of a run of
 // equal k shell first sum their values across the wavefront and the run's
 // last lane adds once (no same-address LDS atomics within the run), 0 = every
@@ -1607,6 +1612,59 @@ static int krows_mode(const JKBuildArgs &a, int k_nao, int l_nao, size_t lds_sta
     return 0;
 }
 
+// LDS K rows of a per-lane launch.  Candidates: the rows over the whole k and
+// l groups and, when the task has kl column windows, over one window, each
+// with and without the density rows.  Ranked by the rows kept in LDS (all
+// four K blocks, then K_il/K_jl only, then none), then by the workgroups a
+// CU can hold with that LDS (hipOccupancyMaxActiveBlocksPerMultiprocessor,
+// which also counts the kernel's registers), then whole groups before
+// windows, then with density rows before without.  kl_win_policy 1 takes the
+// windows whenever their rows fit.
+struct KrChoice { int use_lds; size_t bytes; };
+
+static KrChoice lane_krows_choice(const JKBuildArgs &a, size_t lds_static, bool win_ok,
+                                  const void *kernel, int nfij)
+{
+    KrChoice best{0, 0};
+    long best_key = -1;
+    const bool have_win = a.kl_win && win_ok && LANE_KROWS >= 2;
+    for (int win = 0; win < (have_win ? 2 : 1); ++win) {
+        const int kn = win ? a.kl_win_width : a.k_nao, ln = win ? 0 : a.l_nao;
+        for (int dr = (LANE_DROWS ? 1 : 0); dr >= 0; --dr) {
+            size_t bytes = 0;
+            // the generic krows_mode with this candidate's columns
+            int mode = 0;
+            if (a.vk && a.n_dm == 1 && kn + ln > 0) {
+                const size_t both = (size_t)nfij * (kn + ln) * sizeof(double);
+                const size_t lonly = (size_t)nfij * ln * sizeof(double);
+                if (LANE_KROWS >= 2 && dr && 2 * both + lds_static <= lds_limit()) {
+                    mode = 2 | KR_DROWS; bytes = 2 * both;
+                } else if (LANE_KROWS >= 2 && !dr && both + lds_static <= lds_limit()) {
+                    mode = 2; bytes = both;
+                } else if (!win && LANE_KROWS >= 1 && ln > 0 && lonly + lds_static <= lds_limit()) {
+                    mode = 1; bytes = lonly;
+                }
+            }
+            if (win && (mode & KR_MODE_MASK) != 2) continue;
+            int nb = 1;
+#if LANE_KROWS_OCC
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, LANE_WAVES * WAVE,
+                                                             bytes) != hipSuccess || nb < 1)
+                nb = 1;
+#endif
+            long key = ((long)(mode & KR_MODE_MASK) << 24) | ((long)nb << 8) |
+                       ((win ? 0 : 1) << 1) | (dr ? 1 : 0);
+            if (a.kl_win_policy == 1 && win) key |= 1L << 30;
+            if (key > best_key) {
+                best_key = key;
+                best.use_lds = mode | (win ? KR_WIN : 0);
+                best.bytes = bytes;
+            }
+        }
+    }
+    return best;
+}
+
 template <int LI, int LJ, int LK, int LL>
 struct ClassTraits {
     static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
@@ -1628,29 +1686,23 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-        size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * 2 * WAVE;
-        size_t lds = 0;
-        const int use_lds = krows_mode<LI, LJ>(a, a.k_nao, a.l_nao, lds_static, LANE_KROWS,
-                                               LANE_DROWS, lds);
-        if (a.kl_win && LANE_KROWS >= 2 &&
-            (a.kl_win_policy == 1 || (use_lds & KR_MODE_MASK) != 2)) {
+        const size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * 2 * WAVE;
+        const long nblocks_w = a.kl_win ? (long)a.npairs_ij * a.n_kl_win : 0;
+        const KrChoice c = lane_krows_choice(a, lds_static, nblocks_w <= 0x7fffffffL,
+                                             (const void *)jk_lane_kernel<LI, LJ, LK, LL>,
+                                             ncart(LI) + ncart(LJ));
+        if (c.use_lds & KR_WIN) {
             // K rows over one kl column window per work item
-            size_t lds_w = 0;
-            const int mode_w = krows_mode<LI, LJ>(a, a.kl_win_width, 0, lds_static, LANE_KROWS,
-                                                  LANE_DROWS, lds_w);
-            const long nblocks_w = (long)a.npairs_ij * a.n_kl_win;
-            if ((mode_w & KR_MODE_MASK) == 2 && nblocks_w <= 0x7fffffffL) {
-                JKBuildArgs aw = a;
-                aw.pairs_kl = a.pairs_kl_w;
-                aw.q_kl = a.q_kl_w;
-                aw.pdata_kl = a.pdata_kl_w;
-                hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks_w),
-                                   dim3(LANE_WAVES * WAVE), lds_w, stream, aw, 0, mode_w | KR_WIN);
-                return hipGetLastError();
-            }
+            JKBuildArgs aw = a;
+            aw.pairs_kl = a.pairs_kl_w;
+            aw.q_kl = a.q_kl_w;
+            aw.pdata_kl = a.pdata_kl_w;
+            hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks_w),
+                               dim3(LANE_WAVES * WAVE), c.bytes, stream, aw, 0, c.use_lds);
+        } else {
+            hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks),
+                               dim3(LANE_WAVES * WAVE), c.bytes, stream, a, kl_chunk, c.use_lds);
         }
-        hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks), dim3(LANE_WAVES * WAVE),
-                           lds, stream, a, kl_chunk, use_lds);
     } else {
         using S = TeamShape<LI, LJ, LK, LL>;
         const int kl_chunk = TEAM_KL_CHUNK;

@@ -28,7 +28,7 @@ from gpu4pyscf_amd.lib import clib
 __all__ = ['get_jk', 'get_j', 'get_k', '_VHFOpt']
 
 TILE = 12                 # pair enumeration tile, gpu4pyscf/scf/jk.py:51
-KL_WINDOW_AO = 384        # widest kl column window (AOs of k, and of l)
+KL_WINDOW_AO = 256        # widest kl column window (AOs of k, and of l)
 Q_COND_MARGIN = 4.        # bin width of the descending q_cond ordering
 LMAX = clib.JK_LMAX
 
