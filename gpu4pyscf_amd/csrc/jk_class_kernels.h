@@ -86,6 +86,18 @@
 #ifndef TEAM_KROWS
 #define TEAM_KROWS      0
 #endif
+// team kernel over kl column windows (kl_win[1]) with all four K blocks in
+// LDS, for launches of at least TEAM_KWIN_MIN_ITEMS (ij pair, window) items.
+// Off (used only under kl_win_policy 1): the rows cost more workgroups per
+// CU than the atomics they save.  64-water def2-TZVP build, team kernels on
+// one stream: 1080 ms without; with windows of 64 / 96 / 128 AOs 1159 /
+// 1406 / 1512 ms (32-water: 280 without, 325 with 64-AO windows).
+#ifndef TEAM_KWIN
+#define TEAM_KWIN       0
+#endif
+#ifndef TEAM_KWIN_MIN_ITEMS
+#define TEAM_KWIN_MIN_ITEMS 2048
+#endif
 #ifndef TEAM_MIN_WAVES
 #define TEAM_MIN_WAVES  1
 #endif
@@ -611,15 +623,16 @@ __device__ __forceinline__ WorkItem work_item(int npairs_kl, int kl_chunk)
     return w;
 }
 
-// work items over the kl column windows (window major, ij pair minor); the
-// K-row columns are the window's
+// work items over the kl column windows of kl_win[0] (window major, ij pair
+// minor; the launcher puts the team kernels' set there); the K-row columns
+// are the window's
 __device__ __forceinline__ WorkItem work_item_win(const JKBuildArgs &a)
 {
     WorkItem w;
     const int idx = work_index();
     const int win = idx / a.npairs_ij;
     w.pij = idx - win * a.npairs_ij;
-    const int32_t *d = a.kl_win + 6 * win;
+    const int32_t *d = a.kl_win[0].win + 6 * win;
     w.kl0 = d[0]; w.kl1 = d[1];
     w.k_ao0 = d[2]; w.k_nao = d[3]; w.l_ao0 = d[4]; w.l_nao = d[5];
     return w;
@@ -1627,9 +1640,9 @@ static KrChoice lane_krows_choice(const JKBuildArgs &a, size_t lds_static, bool 
 {
     KrChoice best{0, 0};
     long best_key = -1;
-    const bool have_win = a.kl_win && win_ok && LANE_KROWS >= 2;
+    const bool have_win = a.kl_win[0].win && win_ok && LANE_KROWS >= 2;
     for (int win = 0; win < (have_win ? 2 : 1); ++win) {
-        const int kn = win ? a.kl_win_width : a.k_nao, ln = win ? 0 : a.l_nao;
+        const int kn = win ? a.kl_win[0].width : a.k_nao, ln = win ? 0 : a.l_nao;
         for (int dr = (LANE_DROWS ? 1 : 0); dr >= 0; --dr) {
             size_t bytes = 0;
             // the generic krows_mode with this candidate's columns
@@ -1687,16 +1700,16 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
         const size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * 2 * WAVE;
-        const long nblocks_w = a.kl_win ? (long)a.npairs_ij * a.n_kl_win : 0;
+        const long nblocks_w = a.kl_win[0].win ? (long)a.npairs_ij * a.kl_win[0].n_win : 0;
         const KrChoice c = lane_krows_choice(a, lds_static, nblocks_w <= 0x7fffffffL,
                                              (const void *)jk_lane_kernel<LI, LJ, LK, LL>,
                                              ncart(LI) + ncart(LJ));
         if (c.use_lds & KR_WIN) {
             // K rows over one kl column window per work item
             JKBuildArgs aw = a;
-            aw.pairs_kl = a.pairs_kl_w;
-            aw.q_kl = a.q_kl_w;
-            aw.pdata_kl = a.pdata_kl_w;
+            aw.pairs_kl = a.kl_win[0].pairs;
+            aw.q_kl = a.kl_win[0].q;
+            aw.pdata_kl = a.kl_win[0].pdata;
             hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks_w),
                                dim3(LANE_WAVES * WAVE), c.bytes, stream, aw, 0, c.use_lds);
         } else {
@@ -1713,6 +1726,24 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         size_t lds_static = sizeof(uint32_t) * 2 * S::TB + sizeof(int) * (S::TB / WAVE + 2) +
                             sizeof(double) * (S::NQ * (S::PER_QB + S::DSZ) + S::NFI * S::NFJ) +
                             256;
+        const JKWindows &tw = a.kl_win[1];
+        if (tw.win && a.vk && a.n_dm == 1 && (TEAM_KWIN || a.kl_win_policy == 1)) {
+            // K rows over one kl column window per work item, when that
+            // leaves enough work items to fill the GPU
+            const size_t bytes = sizeof(double) * (S::NFI + S::NFJ) * (size_t)tw.width;
+            const long nblocks_w = (long)a.npairs_ij * tw.n_win;
+            if (bytes + lds_static <= lds_limit() && nblocks_w <= 0x7fffffffL &&
+                (a.kl_win_policy == 1 || nblocks_w >= TEAM_KWIN_MIN_ITEMS)) {
+                JKBuildArgs aw = a;
+                aw.kl_win[0] = tw;
+                aw.pairs_kl = tw.pairs;
+                aw.q_kl = tw.q;
+                aw.pdata_kl = tw.pdata;
+                hipLaunchKernelGGL((jk_team_kernel<LI, LJ, LK, LL>), dim3(nblocks_w), dim3(S::TB),
+                                   bytes, stream, aw, 0, 2 | KR_WIN);
+                return hipGetLastError();
+            }
+        }
         size_t lds = 0;
         const int use_lds = krows_mode<LI, LJ>(a, a.k_nao, a.l_nao, lds_static, TEAM_KROWS,
                                                false, lds);

@@ -322,8 +322,8 @@ int jk_build_tasks(const JKBuildArgs *args, const JKTask *tasks, int ntasks,
         a.pairs_ij = k.pairs_ij; a.pairs_kl = k.pairs_kl;
         a.q_ij = k.q_ij; a.q_kl = k.q_kl;
         a.pdata_ij = k.pdata_ij; a.pdata_kl = k.pdata_kl;
-        a.pairs_kl_w = k.pairs_kl_w; a.q_kl_w = k.q_kl_w; a.pdata_kl_w = k.pdata_kl_w;
-        a.kl_win = k.kl_win; a.n_kl_win = k.n_kl_win; a.kl_win_width = k.kl_win_width;
+        a.kl_win[0] = k.kl_win[0];
+        a.kl_win[1] = k.kl_win[1];
         a.n_quartets = args->n_quartets ? args->n_quartets + t : nullptr;
         a.n_quartets_fp32 = args->n_quartets_fp32 ? args->n_quartets_fp32 + t : nullptr;
         a.quartet_hash = args->quartet_hash ? args->quartet_hash + t : nullptr;
@@ -360,10 +360,12 @@ static int check_args(const JKBuildArgs *args, JKBuildArgs &a, const char *who)
     }
     if (!table_ok(a.rys)) { set_err("%s: Rys table not set", who); return 3; }
     if (!(a.omega >= 0.)) { set_err("%s: omega must be >= 0 (erf attenuation)", who); return 3; }
-    if (a.kl_win && (a.n_kl_win <= 0 || a.kl_win_width <= 0 || !a.pairs_kl_w || !a.q_kl_w ||
-                     !a.pdata_kl_w)) {
-        set_err("%s: kl windows without their pair lists or sizes", who);
-        return 3;
+    for (int t = 0; t < 2; ++t) {
+        const JKWindows &w = a.kl_win[t];
+        if (w.win && (w.n_win <= 0 || w.width <= 0 || !w.pairs || !w.q || !w.pdata)) {
+            set_err("%s: kl windows without their pair lists or sizes", who);
+            return 3;
+        }
     }
     return 0;
 }

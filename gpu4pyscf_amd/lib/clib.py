@@ -14,7 +14,7 @@ _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, 'libjkhip.so')
 TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
 
-JK_ABI_VERSION = 11
+JK_ABI_VERSION = 12
 JK_RYS_NMAX = 11
 JK_BOYS_M = 13
 JK_LMAX = 5
@@ -38,6 +38,12 @@ class JKRysTable(ctypes.Structure):
         ('boys_step', ctypes.c_double),
         ('boys_inv_step', ctypes.c_double),
     ]
+
+
+class JKWindows(ctypes.Structure):
+    '''JKWindows of include/jkhip.h: kl column windows of a pair list.'''
+    _fields_ = [('pairs', c_void_p), ('q', c_void_p), ('pdata', c_void_p), ('win', c_void_p),
+                ('n_win', ctypes.c_int), ('width', ctypes.c_int)]
 
 
 class JKBuildArgs(ctypes.Structure):
@@ -74,9 +80,7 @@ class JKBuildArgs(ctypes.Structure):
         ('omega', ctypes.c_double),
         ('scale', ctypes.c_double),
         ('screen', ctypes.c_int),
-        ('pairs_kl_w', c_void_p), ('q_kl_w', c_void_p), ('pdata_kl_w', c_void_p),
-        ('kl_win', c_void_p),
-        ('n_kl_win', ctypes.c_int), ('kl_win_width', ctypes.c_int),
+        ('kl_win', JKWindows * 2),
         ('kl_win_policy', ctypes.c_int),
     ]
 
@@ -107,9 +111,8 @@ class JKTask(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ('li', 'lj', 'lk', 'll', 'iprim', 'jprim', 'kprim',
                                             'lprim', 'k_ao0', 'k_nao', 'l_ao0', 'l_nao',
                                             'npairs_ij', 'npairs_kl')] + [
-        (n, c_void_p) for n in ('pairs_ij', 'pairs_kl', 'q_ij', 'q_kl', 'pdata_ij', 'pdata_kl',
-                                'pairs_kl_w', 'q_kl_w', 'pdata_kl_w', 'kl_win')] + [
-        ('n_kl_win', ctypes.c_int), ('kl_win_width', ctypes.c_int)]
+        (n, c_void_p) for n in ('pairs_ij', 'pairs_kl', 'q_ij', 'q_kl', 'pdata_ij', 'pdata_kl')] + [
+        ('kl_win', JKWindows * 2)]
 
 
 EXPORTS = ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',

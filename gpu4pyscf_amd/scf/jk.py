@@ -28,7 +28,9 @@ from gpu4pyscf_amd.lib import clib
 __all__ = ['get_jk', 'get_j', 'get_k', '_VHFOpt']
 
 TILE = 12                 # pair enumeration tile, gpu4pyscf/scf/jk.py:51
-KL_WINDOW_AO = 256        # widest kl column window (AOs of k, and of l)
+KL_WINDOW_AO = 256        # widest kl column window (AOs of k, and of l), per-lane kernels
+KL_WINDOW_AO_TEAM = 0     # the same for the team kernels (0: none; they run faster
+                          # with K atomics than with LDS rows over windows)
 Q_COND_MARGIN = 4.        # bin width of the descending q_cond ordering
 LMAX = clib.JK_LMAX
 
@@ -106,15 +108,18 @@ def class_kernel_name(ls):
     return f'{kind}<{ls[0]}, {ls[1]}, {ls[2]}, {ls[3]}>'
 
 
-def _set_kl_win(a, win):
-    '''kl column window fields of a JKTask / JKBuildArgs (win: an entry of
-    _VHFOpt._kl_windows or None).'''
-    if win is None:
-        a.pairs_kl_w = a.q_kl_w = a.pdata_kl_w = a.kl_win = None
-        a.n_kl_win = a.kl_win_width = 0
-    else:
-        a.pairs_kl_w, a.q_kl_w, a.pdata_kl_w, a.kl_win = (x.data_ptr() for x in win[:4])
-        a.n_kl_win, a.kl_win_width = len(win[3]), win[4]
+def _set_kl_win(a, sets):
+    '''kl column windows of a JKTask / JKBuildArgs (sets: an entry of
+    _VHFOpt._kl_windows, [per-lane, team], or None).'''
+    for n in range(2):
+        w = sets[n] if sets is not None else None
+        t = a.kl_win[n]
+        if w is None:
+            t.pairs = t.q = t.pdata = t.win = None
+            t.n_win = t.width = 0
+        else:
+            t.pairs, t.q, t.pdata, t.win = (x.data_ptr() for x in w[:4])
+            t.n_win, t.width = len(w[3]), w[4]
 
 
 def kl_window_table(pairs, offs, ao_loc, nbas, k, l, W):
@@ -183,10 +188,12 @@ class _VHFOpt:
         # optional predicate on (li, lj, lk, ll): the class tasks to run
         self.task_filter = None
         # kl column windows: group pairs wider than kl_window_ao AOs in k or
-        # l are split into windows of at most that many AOs each, so that
-        # the per-lane kernels keep a work item's K rows in LDS (used where
-        # the rows over the whole groups do not fit; kl_win_policy 1: always)
+        # l are split into windows of at most that many AOs each, so that a
+        # work item (ij pair, window) keeps its K rows in LDS; the per-lane
+        # kernels use kl_window_ao, the team kernels kl_window_ao_team (the
+        # launcher decides per launch; kl_win_policy 1: always when they fit)
         self.kl_window_ao = KL_WINDOW_AO
+        self.kl_window_ao_team = KL_WINDOW_AO_TEAM
         self.kl_win_policy = 0
 
     def profile_results(self):
@@ -308,24 +315,29 @@ class _VHFOpt:
         return SortedGTO(self.mol, diffuse_cutoff=0.3)
 
     def _kl_windows(self, cache, dev_pairs, dev_q, dev_pd):
-        '''kl column windows of every group pair wider than kl_window_ao
-        AOs in k or in l: {(k, l): (pairs, q, pdata, table, width)}, the
-        device copies of kl_window_table's reordering and window rows.'''
+        '''kl column windows of every group pair wider than the window in k
+        or in l: {(k, l): [lane windows, team windows]}, each None or
+        (pairs, q, pdata, table, width), the device copies of
+        kl_window_table's reordering and window rows.'''
         torch = _torch()
         mol = self.sorted_mol
         out = {}
-        if int(self.kl_window_ao) <= 0:
-            return out
         for (k, l), (p, _, _) in cache.items():
-            w = kl_window_table(p, mol.group_offsets, mol.ao_loc, mol.nbas, k, l,
-                                int(self.kl_window_ao))
-            if w is None:
-                continue
-            order, tab, width = w
-            dev = dev_pairs[k, l].device
-            o = torch.as_tensor(order, device=dev)
-            out[k, l] = (dev_pairs[k, l][o].contiguous(), dev_q[k, l][o].contiguous(),
-                         dev_pd[k, l][o].contiguous(), torch.as_tensor(tab, device=dev), width)
+            sets = []
+            for W in (int(self.kl_window_ao), int(self.kl_window_ao_team)):
+                w = (kl_window_table(p, mol.group_offsets, mol.ao_loc, mol.nbas, k, l, W)
+                     if W > 0 else None)
+                if w is None:
+                    sets.append(None)
+                    continue
+                order, tab, width = w
+                dev = dev_pairs[k, l].device
+                o = torch.as_tensor(order, device=dev)
+                sets.append((dev_pairs[k, l][o].contiguous(), dev_q[k, l][o].contiguous(),
+                             dev_pd[k, l][o].contiguous(), torch.as_tensor(tab, device=dev),
+                             width))
+            if sets[0] is not None or sets[1] is not None:
+                out[k, l] = sets
         return out
 
     def _order_pairs(self, q, split):

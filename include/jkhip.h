@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  11
+#define JK_ABI_VERSION  12
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         5       /* max angular momentum on the GPU path */
 
@@ -47,6 +47,24 @@ typedef struct {
     int64_t boys;               /* offset of the Boys grid in coef */
     double boys_step, boys_inv_step;
 } JKRysTable;
+
+/* Column windows of a kl pair list: pairs, q and pdata hold the npairs_kl kl
+ * pairs ordered window by window, each window's pairs in the descending q
+ * order of the task's list; window w is win[6w .. 6w+5] = {first pair, end
+ * pair, k_ao0, k_nao, l_ao0, l_nao}: its pairs have their k AOs in
+ * [k_ao0, k_ao0+k_nao) and their l AOs in [l_ao0, l_ao0+l_nao); width = max
+ * over the windows of k_nao + l_nao.  A work item is then (ij pair, window)
+ * and keeps the K rows of its i and j AOs over the window's columns in LDS.
+ * No counterpart in the reference, whose K blocks are accumulated with
+ * global atomics. */
+typedef struct {
+    const uint32_t *pairs;
+    const float *q;
+    const double *pdata;
+    const int32_t *win;             /* device */
+    int n_win;
+    int width;
+} JKWindows;
 
 /* One angular-class task of the J/K build.
  * Replaces RYS_build_jk (gpu4pyscf/lib/gvhf-rys/rys_contract_jk.cu:344-350),
@@ -116,24 +134,12 @@ typedef struct {
      * (RYS_build_j, create_tasks.cu:222-225); JK_SCREEN_K: D_ik, D_jk, D_il
      * or D_jl, vj must be NULL (RYS_build_k, create_tasks.cu:86-93). */
     int screen;
-    /* Column windows of the kl list, for K rows of a work item accumulated
-     * in LDS when the rows over the whole k and l groups exceed the LDS of a
-     * workgroup (NULL / 0: no windows).  pairs_kl_w, q_kl_w and pdata_kl_w
-     * hold the npairs_kl kl pairs ordered window by window, each window's
-     * pairs in the descending q order of pairs_kl; window w is
-     * kl_win[6w .. 6w+5] = {first pair, end pair, k_ao0, k_nao, l_ao0,
-     * l_nao}: its pairs have k in AOs [k_ao0, k_ao0+k_nao) and l in
-     * [l_ao0, l_ao0+l_nao); kl_win_width = max over windows of k_nao + l_nao.
-     * kl_win_policy 0: the per-lane kernels use the windows only when the K
-     * rows over the whole groups do not fit; 1: whenever the window rows fit
-     * (tests).  No counterpart in the reference, whose K rows are
-     * accumulated in registers and global atomics. */
-    const uint32_t *pairs_kl_w;
-    const float *q_kl_w;
-    const double *pdata_kl_w;
-    const int32_t *kl_win;          /* device */
-    int n_kl_win;
-    int kl_win_width;
+    /* Column windows of the kl list (JKWindows), for the K rows of a work
+     * item accumulated in LDS: kl_win[0] for the per-lane kernels, kl_win[1]
+     * (narrower) for the team kernels; a NULL win means none.
+     * kl_win_policy 0: the launcher picks whole groups or windows (see
+     * jk_build); 1: windows whenever their rows fit in LDS (tests). */
+    JKWindows kl_win[2];
     int kl_win_policy;
 } JKBuildArgs;
 
@@ -168,13 +174,7 @@ typedef struct {
     const uint32_t *pairs_ij, *pairs_kl;
     const float *q_ij, *q_kl;
     const double *pdata_ij, *pdata_kl;
-    /* kl column windows, as in JKBuildArgs (NULL / 0: none) */
-    const uint32_t *pairs_kl_w;
-    const float *q_kl_w;
-    const double *pdata_kl_w;
-    const int32_t *kl_win;
-    int n_kl_win;
-    int kl_win_width;
+    JKWindows kl_win[2];            /* as in JKBuildArgs */
 } JKTask;
 
 /* All class tasks of one build: the task loop of gpu4pyscf/scf/jk.py:554-585

@@ -234,8 +234,9 @@ def test_one_sided_builds_match_oracle_sets_and_values(screen, hermi):
 
 @pytest.mark.parametrize('W,with_j', [(12, True), (40, True), (12, False)])
 def test_kl_column_windows_match_oracle(W, with_j):
-    '''kl column windows forced on every per-lane launch that can hold the
-    window's K rows in LDS (kl_win_policy 1, windows of at most W AOs): the
+    '''kl column windows forced on every per-lane and team launch that can
+    hold the window's K rows in LDS (kl_win_policy 1, windows of at most W
+    AOs): the
     surviving quartets of every class task (count and set checksum) equal
     the oracle's, and J/K equal the oracle's to 1e-10, on the 8-water
     def2-SVP cluster (192 AOs; its widest groups have 64 AOs).'''
@@ -244,10 +245,11 @@ def test_kl_column_windows_match_oracle(W, with_j):
     mol, dm = _water8_svp()
     opt = jk._VHFOpt(mol)
     opt.kl_window_ao = W
+    opt.kl_window_ao_team = W
     opt.kl_win_policy = 1
     opt.build()
     assert len(opt._dev['kl_win']) > 0
-    nwin = {k: len(v[3]) for k, v in opt._dev['kl_win'].items()}
+    nwin = {k: len(v[0][3]) for k, v in opt._dev['kl_win'].items() if v[0] is not None}
     print(f'[kl windows W={W} with_j={with_j}] windows per group pair {nwin}', flush=True)
     sm = opt.sorted_mol
     dms = opt.apply_coeff_C_mat_CT(torch.as_tensor(dm, device='cuda')[None])
