@@ -95,6 +95,13 @@
 #ifndef TEAM_KWIN
 #define TEAM_KWIN       0
 #endif
+// smallest workgroup of the team kernel (threads; 64, 128 or 256).  64-water
+// def2-TZVP build, team kernels on one stream: 1082 ms with 64; 256: 1264 ms,
+// 1199-1210 ms with K rows over 96- or 128-AO kl windows; 128 with 96-AO
+// windows: 1176 ms.
+#ifndef TEAM_TB_MIN
+#define TEAM_TB_MIN     64
+#endif
 #ifndef TEAM_KWIN_MIN_ITEMS
 #define TEAM_KWIN_MIN_ITEMS 2048
 #endif
@@ -1187,7 +1194,8 @@ struct TeamShape {
     static constexpr int N = (LI + LJ + LK + LL) / 2 + 1;
     static constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
     static constexpr int TS0 = NROWS > 3 * N ? NROWS : 3 * N;
-    static constexpr int TB = TS0 <= 64 ? 64 : (TS0 <= 128 ? 128 : 256);
+    static constexpr int TB1 = TS0 <= 64 ? 64 : (TS0 <= 128 ? 128 : 256);
+    static constexpr int TB = TB1 < TEAM_TB_MIN ? TEAM_TB_MIN : TB1;
     static constexpr int G_DBL = 2 * N + N * 3 * GS;            // roots + 2D integrals
     static constexpr int S_ROWS0 = 4096 / NFKL;                 // rows staged per pass
     static constexpr int S_ROWS = S_ROWS0 < NROWS ? (S_ROWS0 > 0 ? S_ROWS0 : 1) : NROWS;
