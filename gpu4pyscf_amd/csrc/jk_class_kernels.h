@@ -32,6 +32,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
+#include <type_traits>
 #include "../../include/jkhip.h"
 
 #define PI_FAC          34.98683665524972497    // 2 pi^(5/2)
@@ -50,6 +51,12 @@
 // def2-TZVP build: 95 -> 3261, 120 -> 3271, 170 -> 3134.
 #ifndef LANE_MAX
 #define LANE_MAX        170
+#endif
+// classes whose block does not fit LANE_MAX with its 2D tables run per lane
+// in chunks of rows i when listed in LaneChunked (below), or when they need
+// at most LANE_MAX_CHUNKS chunks (1: the list only); the others on teams
+#ifndef LANE_MAX_CHUNKS
+#define LANE_MAX_CHUNKS 1
 #endif
 // a block size the per-lane kernel leaves to the team kernel (0: none);
 // sending the spilling (ff|ss)/(fs|fs) (100) or (dd|ps)-sized (108) classes
@@ -769,18 +776,18 @@ __device__ __forceinline__ void rys_roots(const JKRysTable &tab, float x, float 
 }
 
 // One quartet per lane in arithmetic type T (double or float): returns the
-// Cartesian block (times the symmetry and normalisation prefactors) in
-// gout[((i*nfj+j)*nfk+k)*nfl+l].
+// rows I0 <= i < I1 of the Cartesian block (times the symmetry and
+// normalisation prefactors) in gout[(((i-I0)*nfj+j)*nfk+k)*nfl+l].
 // pij: records of the (uniform) ij pair; pkl: records of this lane's kl pair.
-template <int LI, int LJ, int LK, int LL, typename T>
+template <int LI, int LJ, int LK, int LL, typename T, int I0 = 0, int I1 = ncart(LI)>
 __device__ __forceinline__ void eri_lane(T *gout, const double *pij, int nij,
                                          const double *pkl, int nkl,
                                          const double *ri, const double *rij,
                                          const double *rk, const double *rkl,
                                          const JKRysTable &rys, double fac_sym, double omega2)
 {
-    constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
-    constexpr int NF = NFI * NFJ * NFK * NFL;
+    constexpr int NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
+    constexpr int NF = (I1 - I0) * NFJ * NFK * NFL;
     constexpr int N = (LI + LJ + LK + LL) / 2 + 1;
     constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
     constexpr CartPow<LI> ci{};
@@ -846,7 +853,7 @@ __device__ __forceinline__ void eri_lane(T *gout, const double *pij, int nij,
                 build_g<LI, LJ, LK, LL>(gz, wt[ir] * fac, PA[2] - rt_aij * PQ[2],
                                         QC[2] + rt_akl * PQ[2], b10, b01, b00, rij_t[2], rkl_t[2]);
 #pragma unroll
-                for (int i = 0; i < NFI; ++i)
+                for (int i = I0; i < I1; ++i)
 #pragma unroll
                 for (int j = 0; j < NFJ; ++j)
 #pragma unroll
@@ -857,12 +864,58 @@ __device__ __forceinline__ void eri_lane(T *gout, const double *pij, int nij,
                     int ix = ci.x[i]*SI + cj.x[j]*SK + ck.x[k]*SJ + cl.x[l];
                     int iy = ci.y[i]*SI + cj.y[j]*SK + ck.y[k]*SJ + cl.y[l];
                     int iz = ci.z[i]*SI + cj.z[j]*SK + ck.z[k]*SJ + cl.z[l];
-                    int n = ((i*NFJ + j)*NFK + k)*NFL + l;
+                    int n = (((i - I0)*NFJ + j)*NFK + k)*NFL + l;
                     gout[n] = fma(gx[ix] * gy[iy], gz[iz], gout[n]);
                 }
             }
         }
     }
+}
+
+// Rows of the Cartesian block a per-lane quartet holds in registers at a
+// time: all NFI when the block and three 2D tables fit in LANE_MAX doubles,
+// else chunks of CH rows i (each chunk re-evaluates the roots and 2D tables).
+template <int LI, int LJ, int LK, int LL>
+struct LaneChunk {
+    static constexpr int NFI = ncart(LI);
+    static constexpr int ROW = ncart(LJ) * ncart(LK) * ncart(LL);
+    static constexpr int GS3 = 3 * (LI+1)*(LJ+1)*(LK+1)*(LL+1);
+    static constexpr int FIT = (LANE_MAX - GS3) / ROW;
+    static constexpr int CH = NFI * ROW + GS3 <= LANE_MAX ? NFI : (FIT > 0 ? FIT : 1);
+    static constexpr int NCH = (NFI + CH - 1) / CH;
+};
+
+// Classes above LANE_MAX that run per lane in chunks: those measured faster
+// than on teams (64-water def2-TZVP build, ms on one stream, team ->
+// chunked lanes: (fp|ds) 91.6 -> 54.4, (fd|ps) 85.0 -> 39.1, (dp|pp) 69.9 ->
+// 52.5, (dd|ds) 37.6 -> 12.7; 32-water alike).  Slower per lane and left on
+// teams: (fp|pp) 70.1 -> 77.2, (fs|dp) 65.9 -> 73.6, (fs|fp) 40.3 -> 68.2,
+// (ds|dd) 8.4 -> 34.9.
+template <int LI, int LJ, int LK, int LL> struct LaneChunked { static constexpr bool ON = false; };
+#define LANE_CHUNKED(a, b, c, d) \
+    template <> struct LaneChunked<a, b, c, d> { static constexpr bool ON = true; };
+LANE_CHUNKED(3, 1, 2, 0) LANE_CHUNKED(3, 2, 1, 0) LANE_CHUNKED(2, 1, 1, 1) LANE_CHUNKED(2, 2, 2, 0)
+#undef LANE_CHUNKED
+
+// f(std::integral_constant<int, c>()) for c = 0 .. NC-1, unrolled at compile time
+template <int C, int NC>
+struct LaneChunkLoop {
+    template <typename F>
+    __device__ __forceinline__ static void run(F &f)
+    {
+        f(std::integral_constant<int, C>());
+        LaneChunkLoop<C + 1, NC>::run(f);
+    }
+};
+template <int NC>
+struct LaneChunkLoop<NC, NC> {
+    template <typename F>
+    __device__ __forceinline__ static void run(F &) {}
+};
+template <int NC, typename F>
+__device__ __forceinline__ void lane_chunks(F &&f)
+{
+    LaneChunkLoop<0, NC>::run(f);
 }
 
 // Evaluates and digests one batch of up to 64 queued quartets (one per lane)
@@ -873,7 +926,7 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                                            int nb, int lane, uint64_t lt_mask, const KRows &kr)
 {
     constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
-    constexpr int NF = NFI * NFJ * NFK * NFL;
+    constexpr int CH = LaneChunk<LI, LJ, LK, LL>::CH;
     const int nbas = a.nbas, nao = a.nao;
     const size_t nao2 = (size_t)nao * nao;
     const bool active = lane < nb;
@@ -889,10 +942,6 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
     const double rk3[3] = {rk[0], rk[1], rk[2]};
     const double rkl[3] = {rl[0] - rk[0], rl[1] - rk[1], rl[2] - rk[2]};
     const int nij = a.iprim * a.jprim, nkl = a.kprim * a.lprim;
-    T gout[NF];
-    eri_lane<LI, LJ, LK, LL, T>(gout, a.pdata_ij + (size_t)pij * nij * PREC, nij,
-                                a.pdata_kl + (size_t)pkl * nkl * PREC, nkl,
-                                sp.ri, sp.rij, rk3, rkl, a.rys, fac_sym, a.omega * a.omega);
     const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
     const int i0 = sp.i0, j0 = sp.j0;
     // runs of equal k shell are contiguous in the queue (tile order):
@@ -907,167 +956,179 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
     const SegMask runs = seg_mask(run, lane);
     // columns of this quartet's k and l AOs in the LDS density rows
     const int dk = k0 - kr.k_ao0, dl = kr.loff + l0 - kr.l_ao0;
-    for (int id = 0; id < a.n_dm; ++id) {
-        const double *D = a.dm + id * nao2;
-        if (a.vk) {
-            double *K = a.vk + id * nao2;
-            // K_ik += (ij|kl) D_jl ; K_il += (ij|kl) D_jk
-#pragma unroll
-            for (int i = 0; i < NFI; ++i) {
-                double kil[NFL], kik[NFK];
-#pragma unroll
-                for (int l = 0; l < NFL; ++l) kil[l] = 0.;
-#pragma unroll
-                for (int k = 0; k < NFK; ++k) kik[k] = 0.;
-                if (active && kr.dstaged) {
-#pragma unroll
-                    for (int j = 0; j < NFJ; ++j)
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l) {
-                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
-                        kil[l] = fma(g, kr.d[(NFI + j) * kr.width + dk + k], kil[l]);
-                        kik[k] = fma(g, kr.d[(NFI + j) * kr.width + dl + l], kik[k]);
-                    }
-                } else if (active) {
-#pragma unroll
-                    for (int j = 0; j < NFJ; ++j)
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l) {
-                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
-                        kil[l] = fma(g, D[(j0+j)*nao + k0+k], kil[l]);
-                        kik[k] = fma(g, D[(j0+j)*nao + l0+l], kik[k]);
-                    }
-                }
-                if (kr.kcols && LANE_KIK_SEGSUM) {
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k) {
-                        double v = seg_sum(kik[k], runs);
-                        if (tail) lds_add(kr.p + i * kr.width + k0 + k - kr.k_ao0, v);
-                    }
-                } else if (kr.kcols) {
-                    if (active) {
-#pragma unroll
+    // rows i of the block in chunks of CH (one chunk unless the block and
+    // its 2D tables would not fit the registers): evaluate, then digest
+    lane_chunks<(NFI + CH - 1) / CH>([&](auto cc) {
+        constexpr int I0 = decltype(cc)::value * CH;
+        constexpr int I1 = I0 + CH < NFI ? I0 + CH : NFI;
+        constexpr int NI = I1 - I0;
+        T gout[NI * NFJ * NFK * NFL];
+        eri_lane<LI, LJ, LK, LL, T, I0, I1>(gout, a.pdata_ij + (size_t)pij * nij * PREC, nij,
+                                            a.pdata_kl + (size_t)pkl * nkl * PREC, nkl,
+                                            sp.ri, sp.rij, rk3, rkl, a.rys, fac_sym,
+                                            a.omega * a.omega);
+        for (int id = 0; id < a.n_dm; ++id) {
+            const double *D = a.dm + id * nao2;
+            if (a.vk) {
+                double *K = a.vk + id * nao2;
+                // K_ik += (ij|kl) D_jl ; K_il += (ij|kl) D_jk
+    #pragma unroll
+                for (int i = I0; i < I1; ++i) {
+                    double kil[NFL], kik[NFK];
+    #pragma unroll
+                    for (int l = 0; l < NFL; ++l) kil[l] = 0.;
+    #pragma unroll
+                    for (int k = 0; k < NFK; ++k) kik[k] = 0.;
+                    if (active && kr.dstaged) {
+    #pragma unroll
+                        for (int j = 0; j < NFJ; ++j)
+    #pragma unroll
                         for (int k = 0; k < NFK; ++k)
-                            lds_add(kr.p + i * kr.width + k0 + k - kr.k_ao0, kik[k]);
+    #pragma unroll
+                        for (int l = 0; l < NFL; ++l) {
+                            double g = gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l];
+                            kil[l] = fma(g, kr.d[(NFI + j) * kr.width + dk + k], kil[l]);
+                            kik[k] = fma(g, kr.d[(NFI + j) * kr.width + dl + l], kik[k]);
+                        }
+                    } else if (active) {
+    #pragma unroll
+                        for (int j = 0; j < NFJ; ++j)
+    #pragma unroll
+                        for (int k = 0; k < NFK; ++k)
+    #pragma unroll
+                        for (int l = 0; l < NFL; ++l) {
+                            double g = gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l];
+                            kil[l] = fma(g, D[(j0+j)*nao + k0+k], kil[l]);
+                            kik[k] = fma(g, D[(j0+j)*nao + l0+l], kik[k]);
+                        }
                     }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k) {
-                        double v = seg_sum(kik[k], runs);
-                        if (tail) atomic_add(K + (i0+i)*nao + k0+k, v);
+                    if (kr.kcols && LANE_KIK_SEGSUM) {
+    #pragma unroll
+                        for (int k = 0; k < NFK; ++k) {
+                            double v = seg_sum(kik[k], runs);
+                            if (tail) lds_add(kr.p + i * kr.width + k0 + k - kr.k_ao0, v);
+                        }
+                    } else if (kr.kcols) {
+                        if (active) {
+    #pragma unroll
+                            for (int k = 0; k < NFK; ++k)
+                                lds_add(kr.p + i * kr.width + k0 + k - kr.k_ao0, kik[k]);
+                        }
+                    } else {
+    #pragma unroll
+                        for (int k = 0; k < NFK; ++k) {
+                            double v = seg_sum(kik[k], runs);
+                            if (tail) atomic_add(K + (i0+i)*nao + k0+k, v);
+                        }
+                    }
+                    if (active) {
+    #pragma unroll
+                        for (int l = 0; l < NFL; ++l) {
+                            if (kr.on) lds_add(kr.p + i * kr.width + kr.loff + l0 + l - kr.l_ao0, kil[l]);
+                            else atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
+                        }
                     }
                 }
-                if (active) {
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l) {
-                        if (kr.on) lds_add(kr.p + i * kr.width + kr.loff + l0 + l - kr.l_ao0, kil[l]);
-                        else atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
+                // K_jk += (ij|kl) D_il ; K_jl += (ij|kl) D_ik
+    #pragma unroll
+                for (int j = 0; j < NFJ; ++j) {
+                    double kjl[NFL], kjk[NFK];
+    #pragma unroll
+                    for (int l = 0; l < NFL; ++l) kjl[l] = 0.;
+    #pragma unroll
+                    for (int k = 0; k < NFK; ++k) kjk[k] = 0.;
+                    if (active && kr.dstaged) {
+    #pragma unroll
+                        for (int i = I0; i < I1; ++i)
+    #pragma unroll
+                        for (int k = 0; k < NFK; ++k)
+    #pragma unroll
+                        for (int l = 0; l < NFL; ++l) {
+                            double g = gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l];
+                            kjl[l] = fma(g, kr.d[i * kr.width + dk + k], kjl[l]);
+                            kjk[k] = fma(g, kr.d[i * kr.width + dl + l], kjk[k]);
+                        }
+                    } else if (active) {
+    #pragma unroll
+                        for (int i = I0; i < I1; ++i)
+    #pragma unroll
+                        for (int k = 0; k < NFK; ++k)
+    #pragma unroll
+                        for (int l = 0; l < NFL; ++l) {
+                            double g = gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l];
+                            kjl[l] = fma(g, D[(i0+i)*nao + k0+k], kjl[l]);
+                            kjk[k] = fma(g, D[(i0+i)*nao + l0+l], kjk[k]);
+                        }
+                    }
+                    if (kr.kcols && LANE_KIK_SEGSUM) {
+    #pragma unroll
+                        for (int k = 0; k < NFK; ++k) {
+                            double v = seg_sum(kjk[k], runs);
+                            if (tail) lds_add(kr.p + (NFI + j) * kr.width + k0 + k - kr.k_ao0, v);
+                        }
+                    } else if (kr.kcols) {
+                        if (active) {
+    #pragma unroll
+                            for (int k = 0; k < NFK; ++k)
+                                lds_add(kr.p + (NFI + j) * kr.width + k0 + k - kr.k_ao0, kjk[k]);
+                        }
+                    } else {
+    #pragma unroll
+                        for (int k = 0; k < NFK; ++k) {
+                            double v = seg_sum(kjk[k], runs);
+                            if (tail) atomic_add(K + (j0+j)*nao + k0+k, v);
+                        }
+                    }
+                    if (active) {
+    #pragma unroll
+                        for (int l = 0; l < NFL; ++l) {
+                            if (kr.on) lds_add(kr.p + (NFI + j) * kr.width + kr.loff + l0 + l - kr.l_ao0, kjl[l]);
+                            else atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
+                        }
                     }
                 }
             }
-            // K_jk += (ij|kl) D_il ; K_jl += (ij|kl) D_ik
-#pragma unroll
-            for (int j = 0; j < NFJ; ++j) {
-                double kjl[NFL], kjk[NFK];
-#pragma unroll
-                for (int l = 0; l < NFL; ++l) kjl[l] = 0.;
-#pragma unroll
-                for (int k = 0; k < NFK; ++k) kjk[k] = 0.;
-                if (active && kr.dstaged) {
-#pragma unroll
-                    for (int i = 0; i < NFI; ++i)
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l) {
-                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
-                        kjl[l] = fma(g, kr.d[i * kr.width + dk + k], kjl[l]);
-                        kjk[k] = fma(g, kr.d[i * kr.width + dl + l], kjk[k]);
-                    }
-                } else if (active) {
-#pragma unroll
-                    for (int i = 0; i < NFI; ++i)
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l) {
-                        double g = gout[((i*NFJ + j)*NFK + k)*NFL + l];
-                        kjl[l] = fma(g, D[(i0+i)*nao + k0+k], kjl[l]);
-                        kjk[k] = fma(g, D[(i0+i)*nao + l0+l], kjk[k]);
-                    }
-                }
-                if (kr.kcols && LANE_KIK_SEGSUM) {
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k) {
-                        double v = seg_sum(kjk[k], runs);
-                        if (tail) lds_add(kr.p + (NFI + j) * kr.width + k0 + k - kr.k_ao0, v);
-                    }
-                } else if (kr.kcols) {
-                    if (active) {
-#pragma unroll
-                        for (int k = 0; k < NFK; ++k)
-                            lds_add(kr.p + (NFI + j) * kr.width + k0 + k - kr.k_ao0, kjk[k]);
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k) {
-                        double v = seg_sum(kjk[k], runs);
-                        if (tail) atomic_add(K + (j0+j)*nao + k0+k, v);
-                    }
-                }
+            if (a.vj) {
+                double *J = a.vj + id * nao2;
+                // J_kl += (ij|kl) D_ij   (per lane)
                 if (active) {
-#pragma unroll
+    #pragma unroll
+                    for (int k = 0; k < NFK; ++k)
+    #pragma unroll
                     for (int l = 0; l < NFL; ++l) {
-                        if (kr.on) lds_add(kr.p + (NFI + j) * kr.width + kr.loff + l0 + l - kr.l_ao0, kjl[l]);
-                        else atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
+                        double s = 0.;
+    #pragma unroll
+                        for (int i = I0; i < I1; ++i)
+    #pragma unroll
+                        for (int j = 0; j < NFJ; ++j)
+                            s = fma((double)gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l],
+                                    D[(i0+i)*nao + j0+j], s);
+                        atomic_add(J + (k0+k)*nao + l0+l, s);
                     }
                 }
-            }
-        }
-        if (a.vj) {
-            double *J = a.vj + id * nao2;
-            // J_kl += (ij|kl) D_ij   (per lane)
-            if (active) {
-#pragma unroll
-                for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                for (int l = 0; l < NFL; ++l) {
+                // J_ij += (ij|kl) D_kl   (common ij: reduce over the wave)
+                double vij[NI*NFJ];
+    #pragma unroll
+                for (int i = I0; i < I1; ++i)
+    #pragma unroll
+                for (int j = 0; j < NFJ; ++j) {
                     double s = 0.;
-#pragma unroll
-                    for (int i = 0; i < NFI; ++i)
-#pragma unroll
-                    for (int j = 0; j < NFJ; ++j)
-                        s = fma((double)gout[((i*NFJ + j)*NFK + k)*NFL + l],
-                                D[(i0+i)*nao + j0+j], s);
-                    atomic_add(J + (k0+k)*nao + l0+l, s);
+                    if (active) {
+    #pragma unroll
+                        for (int k = 0; k < NFK; ++k)
+    #pragma unroll
+                        for (int l = 0; l < NFL; ++l)
+                            s = fma((double)gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l],
+                                    D[(k0+k)*nao + l0+l], s);
+                    }
+                    vij[(i - I0)*NFJ + j] = s;
                 }
+                wave_reduce_scatter<NI*NFJ>(vij, lane, [&](int n, double x) {
+                    atomic_add(J + (i0 + I0 + n / NFJ)*nao + j0 + n % NFJ, x);
+                });
             }
-            // J_ij += (ij|kl) D_kl   (common ij: reduce over the wave)
-            double vij[NFI*NFJ];
-#pragma unroll
-            for (int i = 0; i < NFI; ++i)
-#pragma unroll
-            for (int j = 0; j < NFJ; ++j) {
-                double s = 0.;
-                if (active) {
-#pragma unroll
-                    for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                    for (int l = 0; l < NFL; ++l)
-                        s = fma((double)gout[((i*NFJ + j)*NFK + k)*NFL + l],
-                                D[(k0+k)*nao + l0+l], s);
-                }
-                vij[i*NFJ + j] = s;
-            }
-            wave_reduce_scatter<NFI*NFJ>(vij, lane, [&](int n, double x) {
-                atomic_add(J + (i0 + n / NFJ)*nao + j0 + n % NFJ, x);
-            });
         }
-    }
+    });
 }
 
 // Per-lane kernel.  LANE_WAVES wavefronts share one work item (and its LDS K
@@ -1690,8 +1751,13 @@ template <int LI, int LJ, int LK, int LL>
 struct ClassTraits {
     static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
     static constexpr int GS = (LI+1)*(LJ+1)*(LK+1)*(LL+1);
-    // per-lane path when the block and three 2D tables fit in LANE_MAX doubles
-    static constexpr bool LANE = NF + 3 * GS <= LANE_MAX && NF != LANE_SKIP_NF;
+    // per-lane path when the block and three 2D tables fit in LANE_MAX doubles,
+    // or when chunks of at most LANE_MAX_CHUNKS row blocks of the block do
+    static constexpr bool LANE = (NF + 3 * GS <= LANE_MAX ||
+                                  (LaneChunk<LI, LJ, LK, LL>::FIT > 0 &&
+                                   (LaneChunked<LI, LJ, LK, LL>::ON ||
+                                    LaneChunk<LI, LJ, LK, LL>::NCH <= LANE_MAX_CHUNKS))) &&
+                                 NF != LANE_SKIP_NF;
 };
 
 template <int LI, int LJ, int LK, int LL>

@@ -94,17 +94,24 @@ NREPL = 8
 REPL_BYTES = 16 << 30
 
 
+# classes above LANE_MAX that the per-lane kernel evaluates in chunks of rows
+# (LaneChunked of csrc/jk_class_kernels.h)
+LANE_CHUNKED = {(3, 1, 2, 0), (3, 2, 1, 0), (2, 1, 1, 1), (2, 2, 2, 0)}
+
+
 def class_kernel_name(ls):
     '''Name of the HIP kernel template that evaluates angular class ls =
     (li, lj, lk, ll): the per-lane kernel when the Cartesian block and the
-    three 2D tables fit in LANE_MAX doubles, else the team kernel; classes
-    with a shell of l >= 4 run on the runtime-l kernel.'''
+    three 2D tables fit in LANE_MAX doubles or the class is one of
+    LANE_CHUNKED, else the team kernel; classes with a shell of l >= 4 run
+    on the runtime-l kernel.'''
     if max(ls) > 3:
         return f'jk_generic_kernel<{ls[0]}, {ls[1]}, {ls[2]}, {ls[3]}>'
     nc = [(l + 1) * (l + 2) // 2 for l in ls]
     nf = nc[0] * nc[1] * nc[2] * nc[3]
     gs = (ls[0] + 1) * (ls[1] + 1) * (ls[2] + 1) * (ls[3] + 1)
-    kind = 'jk_lane_kernel' if nf + 3 * gs <= LANE_MAX else 'jk_team_kernel'
+    lane = nf + 3 * gs <= LANE_MAX or tuple(ls) in LANE_CHUNKED
+    kind = 'jk_lane_kernel' if lane else 'jk_team_kernel'
     return f'{kind}<{ls[0]}, {ls[1]}, {ls[2]}, {ls[3]}>'
 
 
