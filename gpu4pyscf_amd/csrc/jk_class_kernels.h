@@ -58,6 +58,12 @@
 #ifndef LANE_MAX_CHUNKS
 #define LANE_MAX_CHUNKS 1
 #endif
+// doubles of one chunk (its rows of the block plus three 2D tables).  220
+// with up to 3 chunks: (fp|pp) 69.9 -> 60.7 ms and (ff|ps) 55.5 -> 50.5 ms on
+// the 64-water build, every class chunked before slower (larger chunks spill)
+#ifndef LANE_CHUNK_MAX
+#define LANE_CHUNK_MAX  LANE_MAX
+#endif
 // a block size the per-lane kernel leaves to the team kernel (0: none);
 // sending the spilling (ff|ss)/(fs|fs) (100) or (dd|ps)-sized (108) classes
 // back to the team kernel measured 658 and 670 ms against 652
@@ -880,7 +886,7 @@ struct LaneChunk {
     static constexpr int NFI = ncart(LI);
     static constexpr int ROW = ncart(LJ) * ncart(LK) * ncart(LL);
     static constexpr int GS3 = 3 * (LI+1)*(LJ+1)*(LK+1)*(LL+1);
-    static constexpr int FIT = (LANE_MAX - GS3) / ROW;
+    static constexpr int FIT = (LANE_CHUNK_MAX - GS3) / ROW;
     static constexpr int CH = NFI * ROW + GS3 <= LANE_MAX ? NFI : (FIT > 0 ? FIT : 1);
     static constexpr int NCH = (NFI + CH - 1) / CH;
 };
@@ -972,30 +978,30 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
             if (a.vk) {
                 double *K = a.vk + id * nao2;
                 // K_ik += (ij|kl) D_jl ; K_il += (ij|kl) D_jk
-    #pragma unroll
+#pragma unroll
                 for (int i = I0; i < I1; ++i) {
                     double kil[NFL], kik[NFK];
-    #pragma unroll
+#pragma unroll
                     for (int l = 0; l < NFL; ++l) kil[l] = 0.;
-    #pragma unroll
+#pragma unroll
                     for (int k = 0; k < NFK; ++k) kik[k] = 0.;
                     if (active && kr.dstaged) {
-    #pragma unroll
+#pragma unroll
                         for (int j = 0; j < NFJ; ++j)
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NFK; ++k)
-    #pragma unroll
+#pragma unroll
                         for (int l = 0; l < NFL; ++l) {
                             double g = gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l];
                             kil[l] = fma(g, kr.d[(NFI + j) * kr.width + dk + k], kil[l]);
                             kik[k] = fma(g, kr.d[(NFI + j) * kr.width + dl + l], kik[k]);
                         }
                     } else if (active) {
-    #pragma unroll
+#pragma unroll
                         for (int j = 0; j < NFJ; ++j)
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NFK; ++k)
-    #pragma unroll
+#pragma unroll
                         for (int l = 0; l < NFL; ++l) {
                             double g = gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l];
                             kil[l] = fma(g, D[(j0+j)*nao + k0+k], kil[l]);
@@ -1003,26 +1009,26 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                         }
                     }
                     if (kr.kcols && LANE_KIK_SEGSUM) {
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NFK; ++k) {
                             double v = seg_sum(kik[k], runs);
                             if (tail) lds_add(kr.p + i * kr.width + k0 + k - kr.k_ao0, v);
                         }
                     } else if (kr.kcols) {
                         if (active) {
-    #pragma unroll
+#pragma unroll
                             for (int k = 0; k < NFK; ++k)
                                 lds_add(kr.p + i * kr.width + k0 + k - kr.k_ao0, kik[k]);
                         }
                     } else {
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NFK; ++k) {
                             double v = seg_sum(kik[k], runs);
                             if (tail) atomic_add(K + (i0+i)*nao + k0+k, v);
                         }
                     }
                     if (active) {
-    #pragma unroll
+#pragma unroll
                         for (int l = 0; l < NFL; ++l) {
                             if (kr.on) lds_add(kr.p + i * kr.width + kr.loff + l0 + l - kr.l_ao0, kil[l]);
                             else atomic_add(K + (i0+i)*nao + l0+l, kil[l]);
@@ -1030,30 +1036,30 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                     }
                 }
                 // K_jk += (ij|kl) D_il ; K_jl += (ij|kl) D_ik
-    #pragma unroll
+#pragma unroll
                 for (int j = 0; j < NFJ; ++j) {
                     double kjl[NFL], kjk[NFK];
-    #pragma unroll
+#pragma unroll
                     for (int l = 0; l < NFL; ++l) kjl[l] = 0.;
-    #pragma unroll
+#pragma unroll
                     for (int k = 0; k < NFK; ++k) kjk[k] = 0.;
                     if (active && kr.dstaged) {
-    #pragma unroll
+#pragma unroll
                         for (int i = I0; i < I1; ++i)
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NFK; ++k)
-    #pragma unroll
+#pragma unroll
                         for (int l = 0; l < NFL; ++l) {
                             double g = gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l];
                             kjl[l] = fma(g, kr.d[i * kr.width + dk + k], kjl[l]);
                             kjk[k] = fma(g, kr.d[i * kr.width + dl + l], kjk[k]);
                         }
                     } else if (active) {
-    #pragma unroll
+#pragma unroll
                         for (int i = I0; i < I1; ++i)
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NFK; ++k)
-    #pragma unroll
+#pragma unroll
                         for (int l = 0; l < NFL; ++l) {
                             double g = gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l];
                             kjl[l] = fma(g, D[(i0+i)*nao + k0+k], kjl[l]);
@@ -1061,26 +1067,26 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                         }
                     }
                     if (kr.kcols && LANE_KIK_SEGSUM) {
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NFK; ++k) {
                             double v = seg_sum(kjk[k], runs);
                             if (tail) lds_add(kr.p + (NFI + j) * kr.width + k0 + k - kr.k_ao0, v);
                         }
                     } else if (kr.kcols) {
                         if (active) {
-    #pragma unroll
+#pragma unroll
                             for (int k = 0; k < NFK; ++k)
                                 lds_add(kr.p + (NFI + j) * kr.width + k0 + k - kr.k_ao0, kjk[k]);
                         }
                     } else {
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NFK; ++k) {
                             double v = seg_sum(kjk[k], runs);
                             if (tail) atomic_add(K + (j0+j)*nao + k0+k, v);
                         }
                     }
                     if (active) {
-    #pragma unroll
+#pragma unroll
                         for (int l = 0; l < NFL; ++l) {
                             if (kr.on) lds_add(kr.p + (NFI + j) * kr.width + kr.loff + l0 + l - kr.l_ao0, kjl[l]);
                             else atomic_add(K + (j0+j)*nao + l0+l, kjl[l]);
@@ -1092,14 +1098,14 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 double *J = a.vj + id * nao2;
                 // J_kl += (ij|kl) D_ij   (per lane)
                 if (active) {
-    #pragma unroll
+#pragma unroll
                     for (int k = 0; k < NFK; ++k)
-    #pragma unroll
+#pragma unroll
                     for (int l = 0; l < NFL; ++l) {
                         double s = 0.;
-    #pragma unroll
+#pragma unroll
                         for (int i = I0; i < I1; ++i)
-    #pragma unroll
+#pragma unroll
                         for (int j = 0; j < NFJ; ++j)
                             s = fma((double)gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l],
                                     D[(i0+i)*nao + j0+j], s);
@@ -1108,15 +1114,15 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                 }
                 // J_ij += (ij|kl) D_kl   (common ij: reduce over the wave)
                 double vij[NI*NFJ];
-    #pragma unroll
+#pragma unroll
                 for (int i = I0; i < I1; ++i)
-    #pragma unroll
+#pragma unroll
                 for (int j = 0; j < NFJ; ++j) {
                     double s = 0.;
                     if (active) {
-    #pragma unroll
+#pragma unroll
                         for (int k = 0; k < NFK; ++k)
-    #pragma unroll
+#pragma unroll
                         for (int l = 0; l < NFL; ++l)
                             s = fma((double)gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l],
                                     D[(k0+k)*nao + l0+l], s);
