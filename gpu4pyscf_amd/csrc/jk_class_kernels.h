@@ -1140,7 +1140,13 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
 // Per-lane kernel.  LANE_WAVES wavefronts share one work item (and its LDS K
 // rows); each wavefront scans every LANE_WAVES-th batch of 64 kl candidates
 // and keeps two queues: quartets whose bound q_ij + q_kl + max(dm_cond)
-// reaches fp32_cut run in FP64, the others in FP32.
+// reaches fp32_cut run in FP64, the others in FP32.  PM (precision mode):
+// 0 both queues, 1 the FP64 queue only, 2 the FP32 queue only (a launch
+// with PM 1 and one with PM 2 together evaluate what one with PM 0 does;
+// each is compiled without the other's arithmetic).  FP64 builds launch PM 1
+// (64-water build 2110 -> 2092 ms against PM 0); mixed builds PM 0, since
+// the pair PM 1 + PM 2 scans and flushes every work item twice (2115 ->
+// 2253 ms; the FP32-only kernels mostly keep the occupancy of PM 0).
 // Minimum waves per SIMD a class kernel is compiled for.  By default the
 // compiler may use all 512 registers (one wave per SIMD); the classes below
 // need a little over 256 and run faster with two waves per SIMD despite a few
@@ -1158,7 +1164,7 @@ CLASS_OCC2(3, 0, 3, 2) CLASS_OCC2(3, 1, 2, 2) CLASS_OCC2(3, 3, 2, 1)
 CLASS_OCC2(3, 2, 3, 0)
 #undef CLASS_OCC2
 
-template <int LI, int LJ, int LK, int LL>
+template <int LI, int LJ, int LK, int LL, int PM>
 __global__ __launch_bounds__(LANE_WAVES * WAVE, (ClassOcc<LI, LJ, LK, LL>::W))
 void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
 {
@@ -1202,6 +1208,9 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                 keep = survives_dmax(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
                                      sp.ish, sp.jsh, nbas, a.dm_cond, a.screen, dmax);
                 lowp = keep && (q_ij + q_kl + dmax < a.fp32_cut);
+                // the quartets of the other precision belong to the other launch
+                if (PM == 1 && lowp) keep = lowp = false;
+                if (PM == 2 && !lowp) keep = false;
                 if (a.quartet_hash && keep) hsum += quartet_mix(bas_ij, bkl);
             }
             const uint64_t m64 = __ballot(keep && !lowp);
@@ -1216,10 +1225,11 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
             wave_sync();
         }
         const bool flush = (base + LANE_WAVES * WAVE >= wi.kl1) || stop;
-        if (qn64 >= WAVE || (flush && qn64 > 0)) {
+        if (PM != 2 && (qn64 >= WAVE || (flush && qn64 > 0))) {
             const int nb = min(qn64, WAVE);
-            lane_batch<LI, LJ, LK, LL, double>(a, sp, bas_ij, wi.pij, queue64, nb, lane,
-                                               lt_mask, kr);
+            if constexpr (PM != 2)
+                lane_batch<LI, LJ, LK, LL, double>(a, sp, bas_ij, wi.pij, queue64, nb, lane,
+                                                   lt_mask, kr);
             wave_sync();
             uint32_t rest = (nb + lane < qn64) ? queue64[nb + lane] : 0u;
             wave_sync();
@@ -1227,10 +1237,11 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
             qn64 -= nb;
             wave_sync();
         }
-        if (qn32 >= WAVE || (flush && qn32 > 0)) {
+        if (PM != 1 && (qn32 >= WAVE || (flush && qn32 > 0))) {
             const int nb = min(qn32, WAVE);
-            lane_batch<LI, LJ, LK, LL, float>(a, sp, bas_ij, wi.pij, queue32, nb, lane,
-                                              lt_mask, kr);
+            if constexpr (PM != 1)
+                lane_batch<LI, LJ, LK, LL, float>(a, sp, bas_ij, wi.pij, queue32, nb, lane,
+                                                  lt_mask, kr);
             wave_sync();
             uint32_t rest = (nb + lane < qn32) ? queue32[nb + lane] : 0u;
             wave_sync();
@@ -1766,6 +1777,32 @@ struct ClassTraits {
                                  NF != LANE_SKIP_NF;
 };
 
+// one per-lane launch of precision mode PM (whole kl chunks or windows, as
+// lane_krows_choice ranks them for that kernel)
+template <int LI, int LJ, int LK, int LL, int PM>
+static hipError_t launch_lane(const JKBuildArgs &a, hipStream_t stream, long nblocks,
+                              int kl_chunk)
+{
+    const size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * 2 * WAVE;
+    const long nblocks_w = a.kl_win[0].win ? (long)a.npairs_ij * a.kl_win[0].n_win : 0;
+    const KrChoice c = lane_krows_choice(a, lds_static, nblocks_w <= 0x7fffffffL,
+                                         (const void *)jk_lane_kernel<LI, LJ, LK, LL, PM>,
+                                         ncart(LI) + ncart(LJ));
+    if (c.use_lds & KR_WIN) {
+        // K rows over one kl column window per work item
+        JKBuildArgs aw = a;
+        aw.pairs_kl = a.kl_win[0].pairs;
+        aw.q_kl = a.kl_win[0].q;
+        aw.pdata_kl = a.kl_win[0].pdata;
+        hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL, PM>), dim3(nblocks_w),
+                           dim3(LANE_WAVES * WAVE), c.bytes, stream, aw, 0, c.use_lds);
+    } else {
+        hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL, PM>), dim3(nblocks),
+                           dim3(LANE_WAVES * WAVE), c.bytes, stream, a, kl_chunk, c.use_lds);
+    }
+    return hipGetLastError();
+}
+
 template <int LI, int LJ, int LK, int LL>
 static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
 {
@@ -1779,23 +1816,11 @@ static hipError_t launch_class(const JKBuildArgs &a, hipStream_t stream)
         nchunks = (a.npairs_kl + kl_chunk - 1) / kl_chunk;
         long nblocks = (long)a.npairs_ij * nchunks;
         if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-        const size_t lds_static = sizeof(uint32_t) * LANE_WAVES * 2 * 2 * WAVE;
-        const long nblocks_w = a.kl_win[0].win ? (long)a.npairs_ij * a.kl_win[0].n_win : 0;
-        const KrChoice c = lane_krows_choice(a, lds_static, nblocks_w <= 0x7fffffffL,
-                                             (const void *)jk_lane_kernel<LI, LJ, LK, LL>,
-                                             ncart(LI) + ncart(LJ));
-        if (c.use_lds & KR_WIN) {
-            // K rows over one kl column window per work item
-            JKBuildArgs aw = a;
-            aw.pairs_kl = a.kl_win[0].pairs;
-            aw.q_kl = a.kl_win[0].q;
-            aw.pdata_kl = a.kl_win[0].pdata;
-            hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks_w),
-                               dim3(LANE_WAVES * WAVE), c.bytes, stream, aw, 0, c.use_lds);
-        } else {
-            hipLaunchKernelGGL((jk_lane_kernel<LI, LJ, LK, LL>), dim3(nblocks),
-                               dim3(LANE_WAVES * WAVE), c.bytes, stream, a, kl_chunk, c.use_lds);
-        }
+        // mixed precision: one kernel with both queues; FP64 only: the kernel
+        // compiled without the FP32 arithmetic
+        if (a.fp32_cut > -1e29f)
+            return launch_lane<LI, LJ, LK, LL, 0>(a, stream, nblocks, kl_chunk);
+        return launch_lane<LI, LJ, LK, LL, 1>(a, stream, nblocks, kl_chunk);
     } else {
         using S = TeamShape<LI, LJ, LK, LL>;
         const int kl_chunk = TEAM_KL_CHUNK;
