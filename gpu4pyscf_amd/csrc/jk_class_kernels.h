@@ -881,12 +881,26 @@ __device__ __forceinline__ void eri_lane(T *gout, const double *pij, int nij,
 // Rows of the Cartesian block a per-lane quartet holds in registers at a
 // time: all NFI when the block and three 2D tables fit in LANE_MAX doubles,
 // else chunks of CH rows i (each chunk re-evaluates the roots and 2D tables).
+// Doubles of one chunk (its rows of the block plus three 2D tables) per
+// class: LANE_CHUNK_MAX, or more where three larger chunks measured faster
+// than five smaller ones (64-water build, ms on one stream: (fp|pp) 77.2 ->
+// 60.7, (ff|ps) -> 50.5 against 55.5 on teams, (fp|fs) on 32 waters 10.6
+// against 11.6 on teams).
+template <int LI, int LJ, int LK, int LL> struct LaneChunkBudget {
+    static constexpr int D = LANE_CHUNK_MAX;
+};
+#define LANE_CHUNK_BUDGET(a, b, c, d, n) \
+    template <> struct LaneChunkBudget<a, b, c, d> { static constexpr int D = n; };
+LANE_CHUNK_BUDGET(3, 1, 1, 1, 220) LANE_CHUNK_BUDGET(3, 3, 1, 0, 220)
+LANE_CHUNK_BUDGET(3, 1, 3, 0, 220)
+#undef LANE_CHUNK_BUDGET
+
 template <int LI, int LJ, int LK, int LL>
 struct LaneChunk {
     static constexpr int NFI = ncart(LI);
     static constexpr int ROW = ncart(LJ) * ncart(LK) * ncart(LL);
     static constexpr int GS3 = 3 * (LI+1)*(LJ+1)*(LK+1)*(LL+1);
-    static constexpr int FIT = (LANE_CHUNK_MAX - GS3) / ROW;
+    static constexpr int FIT = (LaneChunkBudget<LI, LJ, LK, LL>::D - GS3) / ROW;
     static constexpr int CH = NFI * ROW + GS3 <= LANE_MAX ? NFI : (FIT > 0 ? FIT : 1);
     static constexpr int NCH = (NFI + CH - 1) / CH;
 };
@@ -894,13 +908,15 @@ struct LaneChunk {
 // Classes above LANE_MAX that run per lane in chunks: those measured faster
 // than on teams (64-water def2-TZVP build, ms on one stream, team ->
 // chunked lanes: (fp|ds) 91.6 -> 54.4, (fd|ps) 85.0 -> 39.1, (dp|pp) 69.9 ->
-// 52.5, (dd|ds) 37.6 -> 12.7; 32-water alike).  Slower per lane and left on
-// teams: (fp|pp) 70.1 -> 77.2, (fs|dp) 65.9 -> 73.6, (fs|fp) 40.3 -> 68.2,
+// 52.5, (dd|ds) 37.6 -> 12.7, (fp|pp) 70.1 -> 60.7 and (ff|ps) 55.5 -> 50.5
+// with the larger chunks of LaneChunkBudget; 32-water alike).  Slower per
+// lane and left on teams: (fs|dp) 65.9 -> 73.6, (fs|fp) 40.3 -> 68.2,
 // (ds|dd) 8.4 -> 34.9.
 template <int LI, int LJ, int LK, int LL> struct LaneChunked { static constexpr bool ON = false; };
 #define LANE_CHUNKED(a, b, c, d) \
     template <> struct LaneChunked<a, b, c, d> { static constexpr bool ON = true; };
 LANE_CHUNKED(3, 1, 2, 0) LANE_CHUNKED(3, 2, 1, 0) LANE_CHUNKED(2, 1, 1, 1) LANE_CHUNKED(2, 2, 2, 0)
+LANE_CHUNKED(3, 1, 1, 1) LANE_CHUNKED(3, 3, 1, 0) LANE_CHUNKED(3, 1, 3, 0)
 #undef LANE_CHUNKED
 
 // f(std::integral_constant<int, c>()) for c = 0 .. NC-1, unrolled at compile time

@@ -96,7 +96,8 @@ REPL_BYTES = 16 << 30
 
 # classes above LANE_MAX that the per-lane kernel evaluates in chunks of rows
 # (LaneChunked of csrc/jk_class_kernels.h)
-LANE_CHUNKED = {(3, 1, 2, 0), (3, 2, 1, 0), (2, 1, 1, 1), (2, 2, 2, 0)}
+LANE_CHUNKED = {(3, 1, 2, 0), (3, 2, 1, 0), (2, 1, 1, 1), (2, 2, 2, 0), (3, 1, 1, 1),
+                (3, 3, 1, 0), (3, 1, 3, 0)}
 
 
 def class_kernel_name(ls):
