@@ -297,3 +297,28 @@ def test_overlap_mask_drops_only_negligible_pairs():
     assert n_remote > 0
     assert worst_log < prec
     assert worst_remote < 1e3 * math.exp(-50)
+
+
+def test_survivors_within_float32_rounding_of_the_cutoff():
+    '''How many quartets could change sides if q_cond were rounded as the
+    reference's float32 estimator rounds it (nr_sr_estimator.cu:100-397)
+    rather than from FP64: on the 8-water def2-SVP cluster with an
+    incremental density, the quartets whose log bound lies within
+    +-4e-6 (two float32 ulps at |log 1e-13| = 29.9) of the cutoff are at
+    most 1e-4 of the survivors.'''
+    from gpu4pyscf_amd.gto import geometry as G
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    mol = M(atom=G.to_string(G.water_cluster((2, 2, 2), 3.0, seed=0)), basis='def2-svp')
+    sm = SortedGTO(mol, diffuse_cutoff=0.3)
+    cache = O.build_pairs(sm, 1e-13)
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal((sm.nao, sm.nao))
+    D = (1e-3 * (g + g.T) / 2)[None]
+    delta = 4e-6
+    n = {s: len(O.survivors(sm, D, 1, cache, direct_scf_tol=1e-13 * math.exp(s * delta)))
+         for s in (-1, 0, 1)}
+    band = n[-1] - n[1]
+    print(f'8-water def2-SVP: {n[0]} survivors, {band} within +-{delta} of the cutoff')
+    assert n[-1] >= n[0] >= n[1]
+    assert band <= 1e-4 * n[0]
