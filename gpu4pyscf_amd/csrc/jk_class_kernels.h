@@ -1112,6 +1112,26 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
             }
             if (a.vj) {
                 double *J = a.vj + id * nao2;
+                // the D_kl loads of J_ij come before the J_kl atomics: on this GPU
+                // stores and atomics share the load counter, so a load issued
+                // after an atomic waits for it to complete.
+                // J_ij partial sums of this lane (D_kl): reduced over the wave below
+                double vij[NI*NFJ];
+#pragma unroll
+                for (int i = I0; i < I1; ++i)
+#pragma unroll
+                for (int j = 0; j < NFJ; ++j) {
+                    double s = 0.;
+                    if (active) {
+#pragma unroll
+                        for (int k = 0; k < NFK; ++k)
+#pragma unroll
+                        for (int l = 0; l < NFL; ++l)
+                            s = fma((double)gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l],
+                                    D[(k0+k)*nao + l0+l], s);
+                    }
+                    vij[(i - I0)*NFJ + j] = s;
+                }
                 // J_kl += (ij|kl) D_ij   (per lane)
                 if (active) {
 #pragma unroll
@@ -1129,22 +1149,6 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
                     }
                 }
                 // J_ij += (ij|kl) D_kl   (common ij: reduce over the wave)
-                double vij[NI*NFJ];
-#pragma unroll
-                for (int i = I0; i < I1; ++i)
-#pragma unroll
-                for (int j = 0; j < NFJ; ++j) {
-                    double s = 0.;
-                    if (active) {
-#pragma unroll
-                        for (int k = 0; k < NFK; ++k)
-#pragma unroll
-                        for (int l = 0; l < NFL; ++l)
-                            s = fma((double)gout[(((i - I0)*NFJ + j)*NFK + k)*NFL + l],
-                                    D[(k0+k)*nao + l0+l], s);
-                    }
-                    vij[(i - I0)*NFJ + j] = s;
-                }
                 wave_reduce_scatter<NI*NFJ>(vij, lane, [&](int n, double x) {
                     atomic_add(J + (i0 + I0 + n / NFJ)*nao + j0 + n % NFJ, x);
                 });
