@@ -137,8 +137,12 @@ typedef struct {
     /* Column windows of the kl list (JKWindows), for the K rows of a work
      * item accumulated in LDS: kl_win[0] for the per-lane kernels, kl_win[1]
      * (narrower) for the team kernels; a NULL win means none.
-     * kl_win_policy 0: the launcher picks whole groups or windows (see
-     * jk_build); 1: windows whenever their rows fit in LDS (tests). */
+     * kl_win_policy 0: the launcher picks, per launch, whole groups or
+     * windows: first the choice that keeps all four K blocks in LDS, then
+     * the one whose LDS leaves the most workgroups per CU (the team kernels
+     * take kl_win[1] only under policy 1); 1: windows whenever their rows
+     * fit in LDS (tests).  Results do not depend on the choice beyond the
+     * order of the floating-point additions. */
     JKWindows kl_win[2];
     int kl_win_policy;
 } JKBuildArgs;
