@@ -551,26 +551,49 @@ __device__ __forceinline__ void load_shell_pair(ShellPair &s, uint32_t bas_ij, i
 // launch's screen (JKBuildArgs.screen): all six (:155-163), the Coulomb
 // blocks D_ij, D_kl of a J-only build (:222-225) or the exchange blocks
 // D_ik, D_jk, D_il, D_jl of a K-only build (:86-93)
+// swapped (a launch with bra and ket exchanged, JKBuildArgs.swap_ijkl): the
+// test in the task's own order and float32 rounding, with q_ij and cutoff
+// the launch's bra bound and log(direct_scf_tol)
 __device__ __forceinline__ bool survives(float q_kl, float kl_cutoff, float dm_penalty,
                                          uint32_t bas_ij, uint32_t bas_kl, int ish, int jsh,
-                                         int nbas, const float *dm_cond, int screen)
+                                         int nbas, const float *dm_cond, int screen,
+                                         bool swapped = false, float q_ij = 0.f,
+                                         float cutoff = 0.f)
 {
-    if (!(q_kl + dm_penalty >= kl_cutoff && bas_ij >= bas_kl)) return false;
+    if (swapped) {
+        kl_cutoff = cutoff - q_kl;
+        q_kl = q_ij;
+    }
+    if (!(q_kl + dm_penalty >= kl_cutoff && (swapped ? bas_kl >= bas_ij : bas_ij >= bas_kl)))
+        return false;
     int ksh = bas_kl / nbas;
     int lsh = bas_kl - ksh * nbas;
     float d_cutoff = kl_cutoff - q_kl;
     if (screen != JK_SCREEN_K &&
         (dm_cond[bas_ij] > d_cutoff || dm_cond[bas_kl] > d_cutoff))
         return true;
+    // the exchange blocks D_ik, D_jk, D_il, D_jl of the task's shells
+    const int ti = swapped ? ksh : ish, tj = swapped ? lsh : jsh;
+    const int tk = swapped ? ish : ksh, tl = swapped ? jsh : lsh;
     return screen != JK_SCREEN_J &&
-           (dm_cond[ish*nbas+ksh] > d_cutoff ||
-            dm_cond[jsh*nbas+ksh] > d_cutoff ||
-            dm_cond[ish*nbas+lsh] > d_cutoff ||
-            dm_cond[jsh*nbas+lsh] > d_cutoff);
+           (dm_cond[ti*nbas+tk] > d_cutoff ||
+            dm_cond[tj*nbas+tk] > d_cutoff ||
+            dm_cond[ti*nbas+tl] > d_cutoff ||
+            dm_cond[tj*nbas+tl] > d_cutoff);
 }
 
 // SplitMix64 finaliser (Steele, Lea and Flood, OOPSLA 2014; constants of
 // Vigna's splitmix64.c): the per-quartet term of the survivor set checksum
+__device__ __forceinline__ unsigned long long quartet_mix(uint32_t bas_ij, uint32_t bas_kl);
+
+// the checksum term of a quartet in the task's own order (bra and ket of a
+// launch with swap_ijkl exchanged back)
+__device__ __forceinline__ unsigned long long task_mix(const JKBuildArgs &a, uint32_t bas_ij,
+                                                       uint32_t bas_kl)
+{
+    return a.swap_ijkl ? quartet_mix(bas_kl, bas_ij) : quartet_mix(bas_ij, bas_kl);
+}
+
 __device__ __forceinline__ unsigned long long quartet_mix(uint32_t bas_ij, uint32_t bas_kl)
 {
     unsigned long long z = ((unsigned long long)bas_ij << 32) | bas_kl;
@@ -584,18 +607,27 @@ __device__ __forceinline__ unsigned long long quartet_mix(uint32_t bas_ij, uint3
 __device__ __forceinline__ bool survives_dmax(float q_kl, float kl_cutoff, float dm_penalty,
                                               uint32_t bas_ij, uint32_t bas_kl, int ish, int jsh,
                                               int nbas, const float *dm_cond, int screen,
-                                              float &dmax)
+                                              float &dmax, bool swapped = false,
+                                              float q_ij = 0.f, float cutoff = 0.f)
 {
     dmax = -1e30f;
-    if (!(q_kl + dm_penalty >= kl_cutoff && bas_ij >= bas_kl)) return false;
+    if (swapped) {
+        kl_cutoff = cutoff - q_kl;
+        q_kl = q_ij;
+    }
+    if (!(q_kl + dm_penalty >= kl_cutoff && (swapped ? bas_kl >= bas_ij : bas_ij >= bas_kl)))
+        return false;
     int ksh = bas_kl / nbas;
     int lsh = bas_kl - ksh * nbas;
     float d_cutoff = kl_cutoff - q_kl;
     if (screen != JK_SCREEN_K)
         dmax = fmaxf(dm_cond[bas_ij], dm_cond[bas_kl]);
+    // the exchange blocks D_ik, D_jk, D_il, D_jl of the task's shells
+    const int ti = swapped ? ksh : ish, tj = swapped ? lsh : jsh;
+    const int tk = swapped ? ish : ksh, tl = swapped ? jsh : lsh;
     if (screen != JK_SCREEN_J)
-        dmax = fmaxf(dmax, fmaxf(fmaxf(dm_cond[ish*nbas+ksh], dm_cond[jsh*nbas+ksh]),
-                                 fmaxf(dm_cond[ish*nbas+lsh], dm_cond[jsh*nbas+lsh])));
+        dmax = fmaxf(dmax, fmaxf(fmaxf(dm_cond[ti*nbas+tk], dm_cond[tj*nbas+tk]),
+                                 fmaxf(dm_cond[ti*nbas+tl], dm_cond[tj*nbas+tl])));
     return dmax > d_cutoff;
 }
 
@@ -1226,12 +1258,13 @@ void jk_lane_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                 float dmax;
                 const uint32_t bkl = a.pairs_kl[pkl];
                 keep = survives_dmax(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
-                                     sp.ish, sp.jsh, nbas, a.dm_cond, a.screen, dmax);
+                                     sp.ish, sp.jsh, nbas, a.dm_cond, a.screen, dmax,
+                                     a.swap_ijkl != 0, q_ij, a.cutoff);
                 lowp = keep && (q_ij + q_kl + dmax < a.fp32_cut);
                 // the quartets of the other precision belong to the other launch
                 if (PM == 1 && lowp) keep = lowp = false;
                 if (PM == 2 && !lowp) keep = false;
-                if (a.quartet_hash && keep) hsum += quartet_mix(bas_ij, bkl);
+                if (a.quartet_hash && keep) hsum += task_mix(a, bas_ij, bkl);
             }
             const uint64_t m64 = __ballot(keep && !lowp);
             const uint64_t m32 = __ballot(lowp);
@@ -1390,8 +1423,9 @@ void jk_team_kernel(JKBuildArgs a, int kl_chunk, int use_lds)
                 bkl = a.pairs_kl[pkl];
                 fail_margin = q_kl + a.dm_penalty + Q_COND_MARGIN < kl_cutoff;
                 keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl,
-                                sp.ish, sp.jsh, nbas, a.dm_cond, a.screen);
-                if (a.quartet_hash && keep) hsum += quartet_mix(bas_ij, bkl);
+                                sp.ish, sp.jsh, nbas, a.dm_cond, a.screen, a.swap_ijkl != 0,
+                                a.q_ij[wi.pij], a.cutoff);
+                if (a.quartet_hash && keep) hsum += task_mix(a, bas_ij, bkl);
             }
             uint64_t m = __ballot(keep);
             uint64_t mf = __ballot(fail_margin);

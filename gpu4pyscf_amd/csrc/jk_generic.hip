@@ -187,11 +187,12 @@ void jk_generic_kernel(JKBuildArgs a, int kl_chunk, GenExtra ex)
                 fail_margin = q_kl + penalty + Q_COND_MARGIN < kl_cutoff;
                 if (MODE == GEN_JK)
                     keep = survives(q_kl, kl_cutoff, a.dm_penalty, bas_ij, bkl, sp.ish, sp.jsh,
-                                    nbas, a.dm_cond, a.screen);
+                                    nbas, a.dm_cond, a.screen, a.swap_ijkl != 0,
+                                    a.q_ij[wi.pij], a.cutoff);
                 else
                     keep = survives_ejk(q_kl, kl_cutoff, bas_ij, bkl, sp.ish, sp.jsh, nbas,
                                         a.dm_cond, do_j, do_k);
-                if (a.quartet_hash && keep) hsum += quartet_mix(bas_ij, bkl);
+                if (a.quartet_hash && keep) hsum += task_mix(a, bas_ij, bkl);
             }
             const uint64_t m = __ballot(keep);
             if (keep) queue[qn + __popcll(m & lt_mask)] = pkl;

@@ -324,6 +324,7 @@ int jk_build_tasks(const JKBuildArgs *args, const JKTask *tasks, int ntasks,
         a.pdata_ij = k.pdata_ij; a.pdata_kl = k.pdata_kl;
         a.kl_win[0] = k.kl_win[0];
         a.kl_win[1] = k.kl_win[1];
+        a.swap_ijkl = k.swap_ijkl;
         a.n_quartets = args->n_quartets ? args->n_quartets + t : nullptr;
         a.n_quartets_fp32 = args->n_quartets_fp32 ? args->n_quartets_fp32 + t : nullptr;
         a.quartet_hash = args->quartet_hash ? args->quartet_hash + t : nullptr;
@@ -413,6 +414,10 @@ int jk_energy_ip1(const JKBuildArgs *args, double *ejk, double j_factor, double 
         const int st = check_args(args, a, "jk_energy_ip1");
         if (st == -1) return 0;
         if (st) return st;
+    }
+    if (a.swap_ijkl) {
+        set_err("jk_energy_ip1: swap_ijkl is for J/K builds only");
+        return 3;
     }
     if (!ejk || a.n_dm > 2) {
         set_err("jk_energy_ip1: ejk missing or n_dm = %d (1 or 2 densities)", a.n_dm);

@@ -14,7 +14,7 @@ _here = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_here, 'libjkhip.so')
 TABLE_PATH = os.path.join(os.path.dirname(_here), 'csrc', 'rys_table.npz')
 
-JK_ABI_VERSION = 12
+JK_ABI_VERSION = 13
 JK_RYS_NMAX = 11
 JK_BOYS_M = 13
 JK_LMAX = 5
@@ -82,6 +82,7 @@ class JKBuildArgs(ctypes.Structure):
         ('screen', ctypes.c_int),
         ('kl_win', JKWindows * 2),
         ('kl_win_policy', ctypes.c_int),
+        ('swap_ijkl', ctypes.c_int),
     ]
 
 
@@ -112,7 +113,7 @@ class JKTask(ctypes.Structure):
                                             'lprim', 'k_ao0', 'k_nao', 'l_ao0', 'l_nao',
                                             'npairs_ij', 'npairs_kl')] + [
         (n, c_void_p) for n in ('pairs_ij', 'pairs_kl', 'q_ij', 'q_kl', 'pdata_ij', 'pdata_kl')] + [
-        ('kl_win', JKWindows * 2)]
+        ('kl_win', JKWindows * 2), ('swap_ijkl', ctypes.c_int)]
 
 
 EXPORTS = ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define JK_ABI_VERSION  12
+#define JK_ABI_VERSION  13
 #define JK_RYS_NMAX     11      /* max Rys roots: (hh|hh) */
 #define JK_LMAX         5       /* max angular momentum on the GPU path */
 
@@ -145,6 +145,13 @@ typedef struct {
      * order of the floating-point additions. */
     JKWindows kl_win[2];
     int kl_win_policy;
+    /* 1: the task (i, j, k, l) runs with bra and ket exchanged -- li..ll,
+     * the prims, k_ao0..l_nao and the pair lists are those of (k, l, i, j)
+     * -- so that the per-lane J atomics fall on the smaller block; the
+     * canonical test becomes bas_ij <= bas_kl and the quartet checksum keeps
+     * the task's order.  Only for tasks whose two group pairs differ.  No
+     * counterpart in the reference. */
+    int swap_ijkl;
 } JKBuildArgs;
 
 #define JK_SCREEN_JK    0
@@ -179,6 +186,7 @@ typedef struct {
     const float *q_ij, *q_kl;
     const double *pdata_ij, *pdata_kl;
     JKWindows kl_win[2];            /* as in JKBuildArgs */
+    int swap_ijkl;                  /* as in JKBuildArgs */
 } JKTask;
 
 /* All class tasks of one build: the task loop of gpu4pyscf/scf/jk.py:554-585

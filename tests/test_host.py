@@ -93,7 +93,7 @@ def test_jk_build_tasks_validates_on_host():
     from gpu4pyscf_amd.lib import clib
     lib = clib.load()
     assert ctypes.sizeof(clib.JKWindows) == 4 * 8 + 2 * 4
-    assert ctypes.sizeof(clib.JKTask) == 14 * 4 + 6 * 8 + 2 * ctypes.sizeof(clib.JKWindows)
+    assert ctypes.sizeof(clib.JKTask) == 14 * 4 + 6 * 8 + 2 * ctypes.sizeof(clib.JKWindows) + 8
     a = clib.JKBuildArgs()
     a.abi_version = clib.JK_ABI_VERSION
     tasks = (clib.JKTask * 2)()
