@@ -236,3 +236,45 @@ def test_cart2sph_pins():
         M_ = np.array([[sphere_moment(*(np.add(p, q))) for q in comp] for p in comp])
         G_ = c2s(l) @ M_ @ c2s(l).T
         assert abs(G_ - np.eye(2*l+1)).max() < 1e-12
+
+
+def test_fill_task_exchanges_bra_and_ket_of_xs_xp_tasks():
+    '''_VHFOpt._fill_task: the (ps|pp), (ds|dp), (fs|fp) tasks whose group
+    pairs differ get the fields of (k, l, i, j) and swap_ijkl = 1; every other
+    task keeps its own fields; swap_tasks = False exchanges none.'''
+    import torch
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    from gpu4pyscf_amd.lib import clib
+    from gpu4pyscf_amd.scf import jk
+    from tests.molecules import ATOM6
+    sm = SortedGTO(M(atom=ATOM6, basis='def2-tzvp', unit='B'), diffuse_cutoff=0.3)
+    ng = len(sm.l_ctr_counts)
+    keys = [(i, j) for i in range(ng) for j in range(i + 1)]
+    d = {'pairs': {k: torch.zeros(3 + n, dtype=torch.int32) for n, k in enumerate(keys)},
+         'q': {k: torch.zeros(3, dtype=torch.float32) for k in keys},
+         'pdata': {k: torch.zeros(6, dtype=torch.float64) for k in keys},
+         'kl_win': {}}
+    opt = jk._VHFOpt.__new__(jk._VHFOpt)
+    opt.sorted_mol = sm
+    ls = [int(x) for x in sm.uniq_l_ctr[:, 0]]
+    n_swapped = 0
+    for swap_tasks in (True, False):
+        opt.swap_tasks = swap_tasks
+        for (i, j) in keys:
+            for (k, l) in keys:
+                if k > i or (k == i and l > j):
+                    continue
+                a = clib.JKTask()
+                opt._fill_task(a, d, (i, j, k, l))
+                want = (swap_tasks and (i, j) != (k, l) and ls[i] == ls[k] and ls[j] == 0 and
+                        ls[l] == 1)
+                assert a.swap_ijkl == int(want)
+                bra, ket = ((k, l), (i, j)) if want else ((i, j), (k, l))
+                assert (a.li, a.lj, a.lk, a.ll) == (ls[bra[0]], ls[bra[1]], ls[ket[0]], ls[ket[1]])
+                assert a.pairs_ij == d['pairs'][bra].data_ptr()
+                assert a.pairs_kl == d['pairs'][ket].data_ptr()
+                assert a.npairs_ij == len(d['pairs'][bra])
+                assert a.k_ao0 == sm.ao_loc[sm.group_offsets[ket[0]]]
+                n_swapped += want
+    assert n_swapped > 0
