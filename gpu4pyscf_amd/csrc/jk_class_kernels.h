@@ -1205,7 +1205,10 @@ __device__ __forceinline__ void lane_batch(const JKBuildArgs &a, const ShellPair
 // spilled registers (32-water def2-TZVP build, ms, one wave -> two:
 // (fs|ps) 23.1 -> 16.9, (ds|pp) 15.2 -> 13.2, (ds|ds) 9.9 -> 7.1,
 // (dd|pp) 11.6 -> 8.8, (fp|fp) 11.2 -> 8.5, (fd|dp) 9.4 -> 6.2; the classes
-// that need 350 or more registers ran slower with two waves).
+// that need 350 or more registers ran slower with two waves).  One wave per
+// SIMD more for the smallest classes ((ss|ss) 6, (ps|ss) 5, (ps|ps) and
+// (ds|ss) 4, (pp|ss) and (ds|ps) 3) measured slower on the 64-water build
+// ((ps|ss) 119.3 -> 143.7 ms, whole build 2197 -> 2249 ms on one stream).
 template <int LI, int LJ, int LK, int LL> struct ClassOcc { static constexpr int W = 1; };
 #define CLASS_OCC2(a, b, c, d) \
     template <> struct ClassOcc<a, b, c, d> { static constexpr int W = 2; };
@@ -1215,6 +1218,7 @@ CLASS_OCC2(2, 1, 2, 2) CLASS_OCC2(3, 3, 3, 0) CLASS_OCC2(2, 2, 2, 1)
 CLASS_OCC2(3, 0, 3, 2) CLASS_OCC2(3, 1, 2, 2) CLASS_OCC2(3, 3, 2, 1)
 CLASS_OCC2(3, 2, 3, 0)
 #undef CLASS_OCC2
+
 
 template <int LI, int LJ, int LK, int LL, int PM>
 __global__ __launch_bounds__(LANE_WAVES * WAVE, (ClassOcc<LI, LJ, LK, LL>::W))
