@@ -7,15 +7,19 @@
 // quartets and contracts it with the density matrices into J and K.
 //
 // Work decomposition: a work item is (one ij shell pair, one chunk of the kl
-// pair list).  The block scans the kl candidates of its chunk, applies the
-// survival predicate of gpu4pyscf/lib/gvhf-rys/create_tasks.cu:114-180 and
-// compacts the survivors into an LDS queue with wavefront ballots; queued
-// quartets are then evaluated
-//   * by the per-lane kernel (small classes): one quartet per lane, the 2D
-//     Rys integrals and the Cartesian block all in VGPRs;
+// pair list, or one kl column window of it).  The block scans the kl
+// candidates of its chunk, applies the survival predicate of
+// gpu4pyscf/lib/gvhf-rys/create_tasks.cu:114-180 and compacts the survivors
+// into an LDS queue with wavefront ballots; queued quartets are then
+// evaluated
+//   * by the per-lane kernel (small classes, and some d/f classes in chunks of
+//     block rows): one quartet per lane, the 2D Rys integrals and the
+//     Cartesian block (or a chunk of its rows) in VGPRs;
 //   * by the team kernel (large classes): a team of lanes per quartet, the
 //     2D integrals of all roots staged in LDS, each lane owning one (i,j) row
 //     of the block, the block staged through LDS for the contraction.
+// A launch may run its task with bra and ket exchanged (swap_ijkl); the
+// survival test and checksum then follow the task's own order.
 // Digestion follows gpu4pyscf/lib/gvhf-rys/rys_contract_jk.cu:291-330 (four K
 // blocks and two J blocks per quartet, symmetry factors of :156-158,214-220).
 // On this 8-XCD part an FP64 atomic add into HBM is resolved on the memory
