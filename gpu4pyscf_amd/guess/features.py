@@ -49,17 +49,6 @@ def pair_type(ea, eb, same_atom):
     return len(ELEMENTS) + PAIR_TYPES.index(key)
 
 
-def _frame(z, x_hint):
-    z = z / np.linalg.norm(z)
-    x = x_hint - np.dot(x_hint, z) * z
-    if np.linalg.norm(x) < 1e-6:
-        t = np.array([1., 0., 0.]) if abs(z[0]) < .9 else np.array([0., 1., 0.])
-        x = t - np.dot(t, z) * z
-    x /= np.linalg.norm(x)
-    y = np.cross(z, x)
-    return np.stack([x, y, z])          # rows: local axes in lab coordinates
-
-
 def _sh_basis(l, _cache={}):
     '''Sample points on the unit sphere, the monomial -> real-harmonic map
     of angular momentum l, and the least-squares projector onto the
@@ -143,6 +132,21 @@ def _groups(mol, rows):
     return g
 
 
+def _frames(z, x_hint):
+    '''Local frames of many rows at once: (N, 3, 3), rows of each = local
+    x, y, z axes in lab coordinates; z along z, x the part of x_hint normal
+    to z (a lab axis when x_hint is parallel to z).'''
+    z = z / np.linalg.norm(z, axis=1)[:, None]
+    x = x_hint - np.sum(x_hint * z, axis=1)[:, None] * z
+    flat = np.linalg.norm(x, axis=1) < 1e-6
+    if flat.any():
+        t = np.where((np.abs(z[flat, 0]) < .9)[:, None], [1., 0., 0.], [0., 1., 0.])
+        x[flat] = t - np.sum(t * z[flat], axis=1)[:, None] * z[flat]
+    x = x / np.linalg.norm(x, axis=1)[:, None]
+    y = np.cross(z, x)
+    return np.stack([x, y, z], axis=1)
+
+
 def pairs_and_features(mol):
     '''Rows (A, B, type, Q) and the (n_rows, FEAT_PAD) feature matrix.'''
     R = mol.atom_coords() * BOHR
@@ -151,42 +155,54 @@ def pairs_and_features(mol):
         if s not in ELEMENTS:
             raise ValueError(f'learned guess has no parameters for element {s}')
     n = len(R)
+    if n < 2:
+        raise ValueError('learned guess needs at least two atoms for its frames')
+    elem = np.array([ELEMENTS.index(s) for s in sym])
     D = np.linalg.norm(R[:, None] - R[None], axis=2)
+    # environment: radial functions of every other atom within RCUT, summed
+    # per neighbour element (np.add.at adds in the row-major order of the
+    # neighbour list)
+    near = (D < RCUT) & ~np.eye(n, dtype=bool)
+    ia, ic = np.nonzero(near)
     env = np.zeros((n, len(ELEMENTS), NRBF))
-    for a in range(n):
-        for c in range(n):
-            if c != a and D[a, c] < RCUT:
-                env[a, ELEMENTS.index(sym[c])] += rbf(D[a, c])
-    rows = []
-    feats = []
+    np.add.at(env, (ia, elem[ic]), rbf(D[ia, ic]))
+    # rows: A <= B, same atom or R_AB < RCUT, A-major
+    a, b = np.triu_indices(n)
+    keep = (a == b) | (D[a, b] < RCUT)
+    a, b = a[keep], b[keep]
+    same = a == b
+    ptype = np.array([[pair_type(ELEMENTS[i], ELEMENTS[j], False) for j in range(len(ELEMENTS))]
+                      for i in range(len(ELEMENTS))])
+    t = np.where(same, elem[a], ptype[elem[a], elem[b]])
+    nr = len(a)
+    f = np.zeros((nr, FEAT_PAD))
+    f[np.arange(nr), t] = 1.
+    # pair radial functions in the slot of the element pair
+    ps = np.nonzero(~same)[0]
+    slot = N_TYPES + (t[ps] - len(ELEMENTS)) * NRBF
+    f[ps[:, None], slot[:, None] + np.arange(NRBF)] = rbf(D[a[ps], b[ps]])
+    off = N_TYPES + NRBF * len(PAIR_TYPES)
+    f[:, off:off + NRBF * len(ELEMENTS)] = env[a].reshape(nr, -1)
+    f[:, off + NRBF * len(ELEMENTS):off + 2 * NRBF * len(ELEMENTS)] = env[b].reshape(nr, -1)
     # neighbours by distance rounded to 1e-6 Angstrom, ties broken by atom
-    # index, so that equal bond lengths give the same frame after rotation
+    # index, so that equal bond lengths give the same frame after rotation;
+    # each atom itself removed from its list
     Dr = np.round(D, 6)
-    order = np.array([np.lexsort((np.arange(n), Dr[a])) for a in range(n)])
-    for a in range(n):
-        for b in range(a, n):
-            if b != a and D[a, b] >= RCUT:
-                continue
-            same = a == b
-            t = pair_type(sym[a], sym[b], same)
-            f = np.zeros(FEAT_PAD)
-            f[t] = 1.
-            if same:
-                nb = [c for c in order[a] if c != a][:2]
-                z = R[nb[0]] - R[a]
-                x = R[nb[1]] - R[a] if len(nb) > 1 else np.array([1., 0., 0.])
-            else:
-                k = PAIR_TYPES.index(tuple(sorted((sym[a], sym[b]), key=ELEMENTS.index)))
-                f[N_TYPES + k * NRBF:N_TYPES + (k + 1) * NRBF] = rbf(D[a, b])
-                z = R[b] - R[a]
-                nb = [c for c in order[a] if c not in (a, b)]
-                x = R[nb[0]] - R[a] if nb else np.array([1., 0., 0.])
-            off = N_TYPES + NRBF * len(PAIR_TYPES)
-            f[off:off + NRBF * len(ELEMENTS)] = env[a].ravel()
-            f[off + NRBF * len(ELEMENTS):off + 2 * NRBF * len(ELEMENTS)] = env[b].ravel()
-            rows.append((a, b, t, _frame(z, x)))
-            feats.append(f)
-    return rows, np.asarray(feats)
+    order = np.array([np.lexsort((np.arange(n), Dr[i])) for i in range(n)])
+    others = order[order != np.arange(n)[:, None]].reshape(n, n - 1)
+    # same-atom rows: z to the nearest, x to the second nearest neighbour;
+    # pair rows: z from A to B, x to the nearest neighbour of A other than B
+    first = others[a, 0]
+    if n > 2:
+        second = others[a, 1]
+        xi = np.where(same, second, np.where(first == b, second, first))
+        x_hint = R[xi] - R[a]
+    else:
+        x_hint = np.tile([1., 0., 0.], (len(a), 1))     # no third atom
+    z = np.where(same[:, None], R[first] - R[a], R[b] - R[a])
+    Q = _frames(z, x_hint)
+    rows = [(int(i), int(j), int(k), q) for i, j, k, q in zip(a, b, t, Q)]
+    return rows, f
 
 
 def max_block(mol):

@@ -88,3 +88,55 @@ def test_batched_rotations_match_single():
         batch = F._sh_rotations(l, Qs)
         for n in range(5):
             assert np.abs(batch[n] - F._sh_rotation(l, Qs[n])).max() < 1e-13
+
+
+def test_frames_and_layout_follow_the_neighbour_rule():
+    '''Frames written out by hand from the rule (same-atom row: z to the
+    nearest, x to the second nearest neighbour; pair row: z from A to B, x to
+    the nearest neighbour of A other than B; ties by atom index; a lab axis
+    when x is parallel to z) and the feature slots of one pair row.'''
+    from gpu4pyscf_amd.gto.mole import BOHR
+    s2 = np.sqrt(.5)
+    cases = [
+        # bent: O, H1 on z, H2 on y (equal O-H: H1 is the nearest by index)
+        ([[0, 0, 0], [0, 0, 1.8], [0, 1.8, 0]], ['O', 'H', 'H'],
+         {(0, 0): [[0, 1, 0], [-1, 0, 0], [0, 0, 1]],
+          (0, 1): [[0, 1, 0], [-1, 0, 0], [0, 0, 1]],
+          (0, 2): [[0, 0, 1], [1, 0, 0], [0, 1, 0]],
+          (1, 1): [[0, 1, 0], [1, 0, 0], [0, 0, -1]],
+          (1, 2): [[0, -s2, -s2], [-1, 0, 0], [0, s2, -s2]],
+          (2, 2): [[0, 0, 1], [-1, 0, 0], [0, -1, 0]]}),
+        # collinear: x of the O row falls back to the lab x axis
+        ([[0, 0, 0], [0, 0, 1.8], [0, 0, -1.8]], ['O', 'H', 'H'],
+         {(0, 0): [[1, 0, 0], [0, 1, 0], [0, 0, 1]]}),
+        # two atoms along x: no third atom, the lab y axis
+        ([[0, 0, 0], [1.4, 0, 0]], ['H', 'H'],
+         {(0, 0): [[0, 1, 0], [0, 0, 1], [1, 0, 0]],
+          (0, 1): [[0, 1, 0], [0, 0, 1], [1, 0, 0]],
+          (1, 1): [[0, 1, 0], [0, 0, -1], [-1, 0, 0]]}),
+    ]
+    for xyz, sym, want in cases:
+        rows, X = F.pairs_and_features(_mol(np.array(xyz, float), sym))
+        got = {(a, b): Q for a, b, t, Q in rows}
+        n = len(sym)
+        assert list(got) == [(a, b) for a in range(n) for b in range(a, n)]
+        for key, Q in want.items():
+            assert np.allclose(got[key], Q, atol=1e-12), key
+    # feature slots of the O-H1 row of the bent molecule
+    rows, X = F.pairs_and_features(_mol(np.array(cases[0][0], float), cases[0][1]))
+    f = X[1]
+    t = F.N_TYPES - len(F.PAIR_TYPES) + F.PAIR_TYPES.index(('H', 'O'))
+    assert rows[1][2] == t and f[t] == 1. and f[:F.N_TYPES].sum() == 1.
+    k = F.PAIR_TYPES.index(('H', 'O'))
+    assert np.allclose(f[F.N_TYPES + k * F.NRBF:F.N_TYPES + (k + 1) * F.NRBF],
+                       F.rbf(1.8 * BOHR))
+    off = F.N_TYPES + F.NRBF * len(F.PAIR_TYPES)
+    d_hh = np.sqrt(2.) * 1.8 * BOHR
+    env_o = np.zeros((len(F.ELEMENTS), F.NRBF))
+    env_o[F.ELEMENTS.index('H')] = 2 * F.rbf(1.8 * BOHR)
+    env_h = np.zeros((len(F.ELEMENTS), F.NRBF))
+    env_h[F.ELEMENTS.index('O')] = F.rbf(1.8 * BOHR)
+    env_h[F.ELEMENTS.index('H')] = F.rbf(d_hh)
+    assert np.allclose(f[off:off + F.NRBF * 3], env_o.ravel())
+    assert np.allclose(f[off + F.NRBF * 3:off + F.NRBF * 6], env_h.ravel())
+    assert np.all(f[F.FEAT_DIM:] == 0.)
