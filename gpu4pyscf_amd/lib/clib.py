@@ -118,7 +118,7 @@ class JKTask(ctypes.Structure):
 
 EXPORTS = ('jk_build', 'jk_build_tasks', 'jk_pair_data', 'jk_ovlp_mask', 'jk_pair_qcond', 'jk_dm_cond',
            'jk_dm_cond_noalloc', 'jk_energy_ip1', 'jk_md_dm_to_hermite', 'jk_md_hermite_to_j',
-           'jk_md_build_j', 'jk_int1e', 'mlp_gemm_bf16', 'mlp_silu_backward', 'mlp_colsum', 'mlp_mse_grad',
+           'jk_md_build_j', 'jk_int1e', 'jk_ao_transform', 'mlp_gemm_bf16', 'mlp_silu_backward', 'mlp_colsum', 'mlp_mse_grad',
            'mlp_adamw', 'mlp_to_bf16', 'jk_last_error', 'jk_abi_version')
 
 _lib = None
@@ -155,6 +155,8 @@ def load():
     lib.jk_int1e.argtypes = [c_void_p] * 6 + [ctypes.c_int, ctypes.c_int, c_void_p, c_void_p,
                                                ctypes.c_int, JKRysTable, c_void_p]
     L, I, F, P = ctypes.c_long, ctypes.c_int, ctypes.c_float, c_void_p
+    lib.jk_ao_transform.restype = ctypes.c_int
+    lib.jk_ao_transform.argtypes = [P, P, I, I, I, P, P, P, P]
     lib.mlp_gemm_bf16.argtypes = [P, P, P, P, P, I, I, I, L, L, L, L, I, I, P]
     lib.mlp_silu_backward.argtypes = [P, P, P, P, I, I, P]
     lib.mlp_colsum.argtypes = [P, P, I, I, P]

@@ -35,6 +35,37 @@ Q_COND_MARGIN = 4.        # bin width of the descending q_cond ordering
 LMAX = clib.JK_LMAX
 
 
+def csr_rows(mat):
+    '''(ptr, idx, coef) of the nonzeros of a dense host matrix, row by row
+    (int32, int32, float64): row i holds idx[ptr[i]:ptr[i+1]] and their
+    values.'''
+    mat = np.asarray(mat, dtype=np.float64)
+    r, c = np.nonzero(mat)
+    ptr = np.searchsorted(r, np.arange(mat.shape[0] + 1)).astype(np.int32)
+    return ptr, c.astype(np.int32), np.ascontiguousarray(mat[r, c])
+
+
+def _ao_transform(mat, rows):
+    '''out = A M A^T over the last two axes of a float64 device tensor, with A
+    given by its compressed rows and its column count (jk_ao_transform).'''
+    torch = _torch()
+    ptr, idx, coef, n_in = rows
+    n_out = ptr.numel() - 1
+    if mat.dtype != torch.float64 or not mat.is_cuda:
+        raise TypeError('AO transforms take float64 device tensors')
+    if mat.dim() < 2 or tuple(mat.shape[-2:]) != (n_in, n_in):
+        raise ValueError(f'AO transform of a {tuple(mat.shape)} tensor; '
+                         f'the basis has {n_in} AOs')
+    lead = tuple(mat.shape[:-2])
+    src = mat.reshape((-1, n_in, n_in)).contiguous()
+    out = torch.empty(src.shape[:1] + (n_out, n_out), dtype=torch.float64, device=mat.device)
+    if src.shape[0]:
+        clib.check(clib.load().jk_ao_transform(
+            out.data_ptr(), src.data_ptr(), src.shape[0], n_out, n_in, ptr.data_ptr(),
+            idx.data_ptr(), coef.data_ptr(), _stream_ptr()), 'jk_ao_transform')
+    return out.reshape(lead + (n_out, n_out))
+
+
 def _torch():
     import torch
     return torch
@@ -252,6 +283,9 @@ class _VHFOpt:
         d['env'] = torch.as_tensor(mol.env_kernel, device=dev)
         d['ao_loc'] = torch.as_tensor(mol.ao_loc.astype(np.int32), device=dev)
         d['coeff'] = torch.as_tensor(mol.coeff, device=dev)
+        # compressed rows of C (density in) and of C^T (J/K out)
+        for key, mat in (('c_rows', mol.coeff), ('ct_rows', mol.coeff.T)):
+            d[key] = tuple(torch.as_tensor(x, device=dev) for x in csr_rows(mat)) + (mat.shape[1],)
         rys_buf, rys = clib.rys_table(dev)
         d['rys_buf'] = rys_buf
         self._rys = rys
@@ -362,18 +396,18 @@ class _VHFOpt:
         return self.sorted_mol.coeff
 
     def apply_coeff_C_mat_CT(self, mat):
-        '''original AO -> sorted Cartesian AO: C M C^T over the last two axes'''
+        '''original AO -> sorted Cartesian AO: C M C^T over the last two axes
+        (gpu4pyscf/gto/mole.py:906-921), by the sparse transform kernel.'''
         if self._dev is None:
             self.build()
-        C = self._dev['coeff']
-        return C @ mat @ C.T
+        return _ao_transform(mat, self._dev['c_rows'])
 
     def apply_coeff_CT_mat_C(self, mat):
-        '''sorted Cartesian AO -> original AO: C^T M C over the last two axes'''
+        '''sorted Cartesian AO -> original AO: C^T M C over the last two axes
+        (gpu4pyscf/gto/mole.py:923-938), by the sparse transform kernel.'''
         if self._dev is None:
             self.build()
-        C = self._dev['coeff']
-        return C.T @ mat @ C
+        return _ao_transform(mat, self._dev['ct_rows'])
 
     # -- J/K on the sorted basis (gpu4pyscf/scf/jk.py:492-641) -----------
     def tasks(self):

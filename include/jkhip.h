@@ -314,6 +314,20 @@ int jk_dm_cond(float *dm_cond, double *work, const double *dm, int n_dm,
 int jk_dm_cond_noalloc(float *dm_cond, const double *dm, int n_dm, int nao,
                        const int32_t *ao_loc, int nbas, int hermi, void *stream);
 
+/* AO transform between a caller basis and the sorted Cartesian basis:
+ * out[b, i, j] = sum_{p in [ptr[i], ptr[i+1])} sum_{q in [ptr[j], ptr[j+1])}
+ *                coef[p] coef[q] in[b, idx[p], idx[q]]
+ * for b < n_batch, i, j < n_out; in is (n_batch, n_in, n_in) and out
+ * (n_batch, n_out, n_out), row major FP64; (ptr, idx, coef) are the
+ * compressed rows of C (out = C in C^T) or of C^T (out = C^T in C).
+ * Replaces the bra/ket sorted<->spherical/Cartesian passes
+ * (gpu4pyscf/lib/gvhf-rys/mole_helper.cu:1734-1900) behind
+ * SortedGTO.apply_C_mat_CT / apply_CT_mat_C (gpu4pyscf/gto/mole.py:906-938).
+ * n_out <= 262140 and n_batch <= 65535 (status 3 otherwise). */
+int jk_ao_transform(double *out, const double *in, int n_batch, int n_out, int n_in,
+                    const int32_t *ptr, const int32_t *idx, const double *coef,
+                    void *stream);
+
 /* One-electron integrals on the sorted Cartesian basis (nao x nao, written
  * fully): overlap S, kinetic T and nuclear attraction
  * V = -sum_A Z_A <i|1/|r-R_A||j> of point charges.  coords: natm x 3 Bohr,

@@ -278,3 +278,23 @@ def test_fill_task_exchanges_bra_and_ket_of_xs_xp_tasks():
                 assert a.k_ao0 == sm.ao_loc[sm.group_offsets[ket[0]]]
                 n_swapped += want
     assert n_swapped > 0
+
+
+def test_csr_rows_restore_the_dense_matrix():
+    '''The compressed rows behind the sparse AO transforms hold exactly the
+    nonzeros of the coefficient matrix, row by row, for a split and
+    spherical def2-TZVP basis and its transpose.'''
+    from gpu4pyscf_amd.gto.mole import M
+    from gpu4pyscf_amd.gto.sorted_gto import SortedGTO
+    from gpu4pyscf_amd.scf.jk import csr_rows
+    from tests.molecules import ATOM6
+    C = SortedGTO(M(atom=ATOM6, basis='def2-tzvp')).coeff
+    for mat in (C, C.T, np.zeros((3, 4))):
+        ptr, idx, coef = csr_rows(mat)
+        assert ptr.dtype == np.int32 and idx.dtype == np.int32 and coef.dtype == np.float64
+        assert len(ptr) == mat.shape[0] + 1 and ptr[0] == 0 and ptr[-1] == len(idx)
+        back = np.zeros_like(mat)
+        for i in range(mat.shape[0]):
+            back[i, idx[ptr[i]:ptr[i + 1]]] = coef[ptr[i]:ptr[i + 1]]
+        assert np.array_equal(back, mat)
+        assert np.count_nonzero(mat) == len(idx)
