@@ -212,6 +212,30 @@ __device__ __forceinline__ double wave_sum(double v)
 // of 6 M for M separate butterfly sums.  v holds the M values of this lane
 // and is overwritten; `put(n, x)` is called for every n < M, on the lane
 // that holds the total x of value n.
+// Value of the partner lane at the reduce-scatter step of lane bit BIT.
+// Steps inside a 16-lane row use DPP moves (no LDS crossbar): lane ^ 1 and
+// lane ^ 2 by quad permutes, lane ^ 8 by a row rotation of 8, and for
+// BIT = 4 the half-row mirror, lane ^ 7: that partner also has the opposite
+// bit 2 and the same higher bits, and its lanes summed so far are disjoint
+// from this lane's, so the later ^ 2 and ^ 1 steps still cover all eight
+// low-bit patterns.  Steps across rows use the LDS crossbar shuffle.
+#ifndef WAVE_DPP
+#define WAVE_DPP 1
+#endif
+template <int BIT>
+__device__ __forceinline__ double partner_value(double x)
+{
+    constexpr int CTRL = BIT == 1 ? 0xB1 : BIT == 2 ? 0x4E : BIT == 4 ? 0x141 :
+                         BIT == 8 ? 0x128 : -1;
+    if constexpr (WAVE_DPP && CTRL >= 0) {
+        const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
+        return __hiloint2double(hi, lo);
+    } else {
+        return __shfl_xor(x, BIT, WAVE);
+    }
+}
+
 template <int M, int BIT, typename F>
 __device__ __forceinline__ void reduce_scatter_step(double *v, int lane, int base, int nvalid,
                                                     F &put)
@@ -230,7 +254,7 @@ __device__ __forceinline__ void reduce_scatter_step(double *v, int lane, int bas
             const double hi_v = (H + t < M) ? v[H + t] : 0.;
             const double mine = hi ? hi_v : lo_v;
             const double send = hi ? lo_v : hi_v;
-            w[t] = mine + __shfl_xor(send, BIT, WAVE);
+            w[t] = mine + partner_value<BIT>(send);
         }
         const int nv = hi ? (nvalid > H ? nvalid - H : 0) : (nvalid < H ? nvalid : H);
         reduce_scatter_step<H, BIT / 2>(w, lane, hi ? base + H : base, nv, put);
