@@ -218,9 +218,15 @@ __device__ __forceinline__ double wave_sum(double v)
 // BIT = 4 the half-row mirror, lane ^ 7: that partner also has the opposite
 // bit 2 and the same higher bits, and its lanes summed so far are disjoint
 // from this lane's, so the later ^ 2 and ^ 1 steps still cover all eight
-// low-bit patterns.  Steps across rows use the LDS crossbar shuffle.
+// low-bit patterns.  The steps across rows (lane ^ 16, lane ^ 32) use the
+// row-swap moves of gfx950: permlane16_swap(x, x) leaves row r ^ 1's values
+// in its second result on even rows and in its first on odd rows,
+// permlane32_swap(x, x) the other half's in its second result on lanes
+// 0-31 and in its first on lanes 32-63.
+// 0: LDS crossbar shuffles only; 1: DPP for the in-row steps; 2: also the
+// row swaps (64-water build 2017 -> 1998 -> 1989 ms)
 #ifndef WAVE_DPP
-#define WAVE_DPP 1
+#define WAVE_DPP 2
 #endif
 template <int BIT>
 __device__ __forceinline__ double partner_value(double x)
@@ -231,6 +237,18 @@ __device__ __forceinline__ double partner_value(double x)
         const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
         const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
         return __hiloint2double(hi, lo);
+    } else if constexpr (WAVE_DPP > 1 && (BIT == 16 || BIT == 32)) {
+        const bool up = (__lane_id() & BIT) != 0;
+        const int lo = __double2loint(x), hi = __double2hiint(x);
+        if constexpr (BIT == 16) {
+            const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+            const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+            return __hiloint2double(up ? (int)b[0] : (int)b[1], up ? (int)a[0] : (int)a[1]);
+        } else {
+            const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+            const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+            return __hiloint2double(up ? (int)b[0] : (int)b[1], up ? (int)a[0] : (int)a[1]);
+        }
     } else {
         return __shfl_xor(x, BIT, WAVE);
     }
